@@ -1,0 +1,188 @@
+/*
+ * scattersearch.h - C ABI of the MI355X graph-ANN query path.
+ *
+ * Replaces the call surface of the reference's class ANNSearch
+ * (include/annsearch.h:37-84) and its file utilities (include/util.h:36-41,
+ * src/util.cpp:40-93).  Plain pointers and sizes only; every function returns
+ * SS_OK (0) or a negative status and never exits the process; the message of
+ * the last failure on the calling thread is ss_last_error().
+ *
+ * Data layout on the device (one index = one GPU):
+ *   base vectors  fp32 [num][dim_pad], dim_pad = dim rounded up to 8,
+ *                 zero-filled past dim, rows 32-byte aligned;
+ *   adjacency     u32 [num][r_pad], each row prefix-filled with neighbour
+ *                 ids and padded with SS_NONE, r_pad <= 64.
+ */
+#ifndef SCATTERSEARCH_H
+#define SCATTERSEARCH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_OK 0
+#define SS_EINVAL (-22)  /* bad argument */
+#define SS_ENOMEM (-12)  /* host or device allocation failed */
+#define SS_EIO (-5)      /* file missing, short or malformed */
+#define SS_EDEVICE (-19) /* no usable GPU or a HIP call failed */
+#define SS_ESTATE (-71)  /* call not valid in the index's current state */
+#define SS_EBUSY (-16)   /* serving queue full */
+
+#define SS_NONE 0xFFFFFFFFu
+
+/* enum Metric (include/annsearch.h:22-26): L2 is the squared distance,
+ * INNER_PRODUCT the negated dot product, both "smaller is better". */
+#define SS_METRIC_L2 0
+#define SS_METRIC_INNER_PRODUCT 1
+
+/* Search launch modes. */
+#define SS_MODE_BATCHED 0   /* one persistent launch fed from a device queue */
+#define SS_MODE_PER_QUERY 1 /* one launch per query (the plain baseline) */
+
+#define SS_MAX_L 512
+#define SS_MAX_K 512
+#define SS_MAX_RPAD 64
+
+typedef struct ss_index ss_index;
+
+typedef struct ss_search_params {
+    uint32_t mode;            /* SS_MODE_* */
+    uint32_t waves_per_block; /* queries resident per workgroup; 0 = tuned */
+    uint32_t visited_slots;   /* LDS visited-hash slots per query (power of
+                                 two, 256..32768); 0 = tuned */
+    uint32_t blocks;          /* persistent grid size; 0 = enough to fill the GPU */
+    uint32_t use_graph;       /* 1: capture the batched launch in a hipGraph
+                                 and replay it for identical shapes */
+    uint32_t reserved[3];
+} ss_search_params;
+
+/* Per-query counters written by every search. */
+typedef struct ss_query_stats {
+    uint32_t ndc;   /* distance computations, including the seed ones */
+    uint32_t hops;  /* expanded nodes */
+    uint32_t lossy; /* visited-hash inserts that found no free slot */
+    uint32_t reserved;
+} ss_query_stats;
+
+const char *ss_last_error(void);
+
+/* ANNSearch::ANNSearch(dim, num, base, Metric) (src/annsearch.cpp:11-58).
+ * Unlike the reference, which borrows `base`, the index copies it into HBM
+ * and owns that copy; `base` is row-major [num][dim] on the host. */
+int ss_create(uint32_t dim, uint32_t num, const float *base, int metric, ss_index **out);
+
+/* Same, with the base already on the device as [num][dim_pad] fp32
+ * (ss_dim_padded); the index copies it. */
+int ss_create_from_device(uint32_t dim, uint32_t num, const float *d_base, int metric,
+                          ss_index **out);
+
+int ss_destroy(ss_index *index);
+
+uint32_t ss_dim_padded(uint32_t dim);
+int ss_index_info(const ss_index *index, uint32_t *dim, uint32_t *num, uint32_t *r_pad,
+                  uint32_t *ep, int *metric);
+
+/* ANNSearch::LoadGraph (src/annsearch.cpp:64-80): NSG file
+ * [u32 width][u32 ep] then per node [u32 deg][deg x u32 ids] until EOF.
+ * Unlike the reference, a missing or malformed file, an id >= num, a node
+ * count different from num or a degree above SS_MAX_RPAD is an error.
+ * Repeated ids inside one row are dropped (the reference's visited set
+ * skips them anyway). */
+int ss_load_graph(ss_index *index, const char *nsg_path);
+
+/* Install a graph from host memory: adj is [num][r_pad] u32, rows
+ * prefix-filled and padded with SS_NONE. */
+int ss_set_graph(ss_index *index, const uint32_t *adj, uint32_t r_pad, uint32_t ep);
+
+/* Copy the installed graph back: adj_out has room for num * r_pad. */
+int ss_get_graph(const ss_index *index, uint32_t *adj_out, uint32_t *r_pad, uint32_t *ep);
+
+/* ANNSearch::SearchArraySimulation (src/annsearch.cpp:182-259) and its
+ * distance-returning form SearchArraySimulationForPipeline (:261-324) for a
+ * batch of queries: the best-first search from the graph's entry point with
+ * pool size L[q] returning the K[q] nearest found.  queries is [nq][dim] on
+ * the host; K and L may be NULL (then every query uses k_default /
+ * l_default).  Outputs are [nq][k_stride]; entries past K[q], or past the
+ * filled pool, are (SS_NONE, +inf).  stats may be NULL. */
+int ss_search(ss_index *index, const float *queries, uint32_t nq,
+              const uint32_t *K, const uint32_t *L, uint32_t k_default, uint32_t l_default,
+              uint32_t k_stride, const ss_search_params *params,
+              uint32_t *out_ids, float *out_dists, ss_query_stats *stats);
+
+/* Same with every buffer already on the device: d_queries is
+ * [nq][dim_pad]; k_max / l_max bound K[q] / L[q] (they size on-chip state);
+ * stream is a hipStream_t (NULL = default stream).  Asynchronous: returns
+ * after enqueueing. */
+int ss_search_device(ss_index *index, const float *d_queries, uint32_t nq,
+                     const uint32_t *d_K, const uint32_t *d_L, uint32_t k_max, uint32_t l_max,
+                     uint32_t k_stride, const ss_search_params *params,
+                     uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats,
+                     void *stream);
+
+/* Exact k nearest base points per query by (distance, id), distances in
+ * the same arithmetic as the search (ground truth; load_groundtruth files,
+ * src/util.cpp:72-93, hold its ids).  exclude (optional, host, [nq]) is an
+ * id skipped for each query. */
+int ss_bruteforce(ss_index *index, const float *queries, uint32_t nq, uint32_t k,
+                  const uint32_t *exclude, uint32_t *out_ids, float *out_dists);
+
+/* Device form: d_queries [nq][dim_pad]; exclude_self != 0 skips base id
+ * (query_base_offset + q) for query q (k-NN of the base itself). */
+int ss_bruteforce_device(ss_index *index, const float *d_queries, uint32_t nq, uint32_t k,
+                         int exclude_self, uint32_t query_base_offset,
+                         uint32_t *d_out_ids, float *d_out_dists, void *stream);
+
+/* k-NN graph of the base itself (each point's k nearest other points),
+ * written to host arrays [num][k]. */
+int ss_knn_graph(ss_index *index, uint32_t k, uint32_t *out_ids, float *out_dists);
+
+/* ---------------------------------------------------------------------- */
+/* Host-side files and graph construction (no GPU needed).                  */
+/* ---------------------------------------------------------------------- */
+
+/* load_fbin (src/util.cpp:40-70): [u32 num][u32 dim][num*dim f32].
+ * ss_fbin_header reads the sizes; ss_fbin_read fills data [num][dim]. */
+int ss_fbin_header(const char *path, uint32_t *num, uint32_t *dim);
+int ss_fbin_read(const char *path, float *data, uint32_t num, uint32_t dim);
+int ss_fbin_write(const char *path, const float *data, uint32_t num, uint32_t dim);
+
+/* load_groundtruth (src/util.cpp:72-93): [u32 nq][u32 k][nq*k u32]. */
+int ss_gt_header(const char *path, uint32_t *nq, uint32_t *k);
+int ss_gt_read(const char *path, uint32_t *ids, uint32_t nq, uint32_t k);
+int ss_gt_write(const char *path, const uint32_t *ids, uint32_t nq, uint32_t k);
+
+/* NSG graph file (src/annsearch.cpp:64-80) to/from the padded layout.
+ * ss_nsg_header scans the file: node count, largest degree, entry point. */
+int ss_nsg_header(const char *path, uint32_t *num, uint32_t *max_degree, uint32_t *ep);
+int ss_nsg_read(const char *path, uint32_t *adj, uint32_t num, uint32_t r_pad);
+int ss_nsg_write(const char *path, const uint32_t *adj, uint32_t num, uint32_t r_pad,
+                 uint32_t ep);
+
+/* Navigating node: the base point nearest (L2) to the centroid. */
+int ss_medoid(const float *base, uint32_t num, uint32_t dim, uint32_t *ep);
+
+/* NSG-style graph from candidate lists: for every node, candidates =
+ * knn[v] (knn_k ids) plus extra[v] (extra_k ids, e.g. the pool of a search
+ * for v from the entry point; may be NULL), pruned by the occlusion rule to
+ * at most R edges (c is dropped when a kept s has alpha * d(s,c) < d(v,c);
+ * alpha = 1 is MRNG), then reverse edges are added (re-pruning a node that
+ * overflows), then every node not reachable from ep is attached to its
+ * nearest reachable candidate that has a free slot.  adj_out is [num][R]
+ * padded with SS_NONE.  Distances use the metric of the search. */
+int ss_build_graph(const float *base, uint32_t num, uint32_t dim, int metric,
+                   const uint32_t *knn, uint32_t knn_k,
+                   const uint32_t *extra, uint32_t extra_k,
+                   uint32_t R, float alpha, uint32_t ep, int nthreads, uint32_t *adj_out,
+                   uint32_t *unreachable_fixed);
+
+/* Number of nodes reachable from ep (breadth-first over adj). */
+int ss_graph_reachable(const uint32_t *adj, uint32_t num, uint32_t r_pad, uint32_t ep,
+                       uint32_t *reachable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,746 @@
+// C ABI of the graph-ANN query path: index lifetime, graph install, search
+// and brute-force launches.  Mirrors the reference class ANNSearch
+// (include/annsearch.h:37-84); every entry point returns a status instead of
+// exiting and records a message for ss_last_error().
+#include "ss_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace ss {
+
+size_t bruteforce_lds_bytes(uint32_t dim_pad);
+uint32_t bruteforce_kmax();
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+}  // namespace ss
+
+using ss::set_error;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= cap)
+            return hipSuccess;
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr), std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess)
+            cap = n;
+        return e;
+    }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? SS_ENOMEM : SS_EDEVICE;
+}
+
+}  // namespace
+
+struct ss_index {
+    uint32_t dim = 0, dim_pad = 0, num = 0;
+    int metric = SS_METRIC_L2;
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_cu = 0;
+    float *d_base = nullptr;
+    uint32_t *d_adj = nullptr;
+    uint32_t r_pad = 0, ep = 0;
+    bool has_graph = false;
+    uint32_t *d_queue = nullptr;
+    hipStream_t stream = nullptr;
+    std::mutex mu;  // serialises the host-buffer entry points
+    DevBuf<float> q;
+    DevBuf<uint32_t> kl;
+    DevBuf<uint32_t> ids;
+    DevBuf<float> dists;
+    DevBuf<ss_query_stats> stats;
+    // hipGraph cache of the last batched launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    ss::SearchArgs graph_args{};
+    uint32_t graph_blocks = 0, graph_wpb = 0;
+    hipStream_t graph_stream = nullptr;
+};
+
+extern "C" {
+
+const char *ss_last_error(void) { return ss::g_error.c_str(); }
+
+uint32_t ss_dim_padded(uint32_t dim) { return (dim + 7u) & ~7u; }
+
+static int init_device(ss_index *ix)
+{
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        set_error("no HIP device available");
+        return SS_EDEVICE;
+    }
+    if ((e = hipGetDevice(&ix->device)) != hipSuccess)
+        return hip_fail(e, "hipGetDevice");
+    int cus = 0, lds = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ix->device)) != hipSuccess)
+        return hip_fail(e, "hipDeviceGetAttribute(cu count)");
+    if ((e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ix->device)) !=
+        hipSuccess)
+        return hip_fail(e, "hipDeviceGetAttribute(lds)");
+    ix->num_cus = cus;
+    ix->lds_per_cu = (size_t)lds;
+    if ((e = ss::init_kernel_attributes(ix->lds_per_cu)) != hipSuccess)
+        return hip_fail(e, "hipFuncSetAttribute");
+    if ((e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate");
+    if ((e = hipMalloc(reinterpret_cast<void **>(&ix->d_queue), 64)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(queue)");
+    return SS_OK;
+}
+
+static int validate_create(uint32_t dim, uint32_t num, const void *base, int metric, ss_index **out)
+{
+    if (!out) {
+        set_error("ss_create: out is NULL");
+        return SS_EINVAL;
+    }
+    *out = nullptr;
+    if (!base || num == 0 || dim == 0) {
+        set_error("ss_create: empty base");
+        return SS_EINVAL;
+    }
+    if (metric != SS_METRIC_L2 && metric != SS_METRIC_INNER_PRODUCT) {
+        set_error("ss_create: unknown metric");  // src/annsearch.cpp:52-56
+        return SS_EINVAL;
+    }
+    if (!ss::dim_supported(ss_dim_padded(dim))) {
+        set_error("ss_create: dimension out of range (1..4096)");
+        return SS_EINVAL;
+    }
+    if (num >= 0x7FFFFFFFu) {
+        set_error("ss_create: more than 2^31-2 points");
+        return SS_EINVAL;
+    }
+    return SS_OK;
+}
+
+static int create_common(uint32_t dim, uint32_t num, int metric, ss_index **out, ss_index **ixp)
+{
+    ss_index *ix = new (std::nothrow) ss_index();
+    if (!ix) {
+        set_error("ss_create: out of host memory");
+        return SS_ENOMEM;
+    }
+    ix->dim = dim;
+    ix->dim_pad = ss_dim_padded(dim);
+    ix->num = num;
+    ix->metric = metric;
+    int rc = init_device(ix);
+    if (rc) {
+        ss_destroy(ix);
+        return rc;
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&ix->d_base), (size_t)num * ix->dim_pad * sizeof(float));
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "hipMalloc(base)");
+        ss_destroy(ix);
+        return rc;
+    }
+    *ixp = ix;
+    (void)out;
+    return SS_OK;
+}
+
+int ss_create(uint32_t dim, uint32_t num, const float *base, int metric, ss_index **out)
+{
+    int rc = validate_create(dim, num, base, metric, out);
+    if (rc)
+        return rc;
+    ss_index *ix = nullptr;
+    if ((rc = create_common(dim, num, metric, out, &ix)))
+        return rc;
+    hipError_t e;
+    if (ix->dim_pad == dim) {
+        e = hipMemcpy(ix->d_base, base, (size_t)num * dim * sizeof(float), hipMemcpyHostToDevice);
+    } else {
+        e = hipMemcpy2D(ix->d_base, ix->dim_pad * sizeof(float), base, dim * sizeof(float), dim * sizeof(float),
+                        num, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemset2D(reinterpret_cast<char *>(ix->d_base) + dim * sizeof(float), ix->dim_pad * sizeof(float),
+                            0, (ix->dim_pad - dim) * sizeof(float), num);
+    }
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "upload of base vectors");
+        ss_destroy(ix);
+        return rc;
+    }
+    *out = ix;
+    return SS_OK;
+}
+
+int ss_create_from_device(uint32_t dim, uint32_t num, const float *d_base, int metric, ss_index **out)
+{
+    int rc = validate_create(dim, num, d_base, metric, out);
+    if (rc)
+        return rc;
+    ss_index *ix = nullptr;
+    if ((rc = create_common(dim, num, metric, out, &ix)))
+        return rc;
+    hipError_t e = hipMemcpy(ix->d_base, d_base, (size_t)num * ix->dim_pad * sizeof(float), hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "copy of base vectors");
+        ss_destroy(ix);
+        return rc;
+    }
+    *out = ix;
+    return SS_OK;
+}
+
+int ss_destroy(ss_index *ix)
+{
+    if (!ix)
+        return SS_OK;
+    if (ix->graph_exec)
+        (void)hipGraphExecDestroy(ix->graph_exec);
+    if (ix->graph)
+        (void)hipGraphDestroy(ix->graph);
+    if (ix->stream)
+        (void)hipStreamSynchronize(ix->stream);
+    if (ix->d_base)
+        (void)hipFree(ix->d_base);
+    if (ix->d_adj)
+        (void)hipFree(ix->d_adj);
+    if (ix->d_queue)
+        (void)hipFree(ix->d_queue);
+    ix->q.release();
+    ix->kl.release();
+    ix->ids.release();
+    ix->dists.release();
+    ix->stats.release();
+    if (ix->stream)
+        (void)hipStreamDestroy(ix->stream);
+    delete ix;
+    return SS_OK;
+}
+
+int ss_index_info(const ss_index *ix, uint32_t *dim, uint32_t *num, uint32_t *r_pad, uint32_t *ep, int *metric)
+{
+    if (!ix) {
+        set_error("ss_index_info: NULL index");
+        return SS_EINVAL;
+    }
+    if (dim)
+        *dim = ix->dim;
+    if (num)
+        *num = ix->num;
+    if (r_pad)
+        *r_pad = ix->r_pad;
+    if (ep)
+        *ep = ix->ep;
+    if (metric)
+        *metric = ix->metric;
+    return SS_OK;
+}
+
+int ss_set_graph(ss_index *ix, const uint32_t *adj, uint32_t r_pad, uint32_t ep)
+{
+    if (!ix || !adj) {
+        set_error("ss_set_graph: NULL argument");
+        return SS_EINVAL;
+    }
+    if (r_pad == 0 || r_pad > SS_MAX_RPAD) {
+        set_error("ss_set_graph: row width must be 1..64");
+        return SS_EINVAL;
+    }
+    if (ep >= ix->num) {
+        set_error("ss_set_graph: entry point out of range");
+        return SS_EINVAL;
+    }
+    // Validate: ids < num, rows prefix-filled; drop repeated ids in a row.
+    std::vector<uint32_t> clean((size_t)ix->num * r_pad, SS_NONE);
+    for (uint32_t n = 0; n < ix->num; n++) {
+        const uint32_t *row = adj + (size_t)n * r_pad;
+        uint32_t *dst = clean.data() + (size_t)n * r_pad;
+        uint32_t w = 0;
+        bool ended = false;
+        for (uint32_t j = 0; j < r_pad; j++) {
+            const uint32_t id = row[j];
+            if (id == SS_NONE) {
+                ended = true;
+                continue;
+            }
+            if (ended) {
+                set_error("ss_set_graph: row " + std::to_string(n) + " has an id after padding");
+                return SS_EINVAL;
+            }
+            if (id >= ix->num) {
+                set_error("ss_set_graph: row " + std::to_string(n) + " holds id " + std::to_string(id) +
+                          " >= num");
+                return SS_EINVAL;
+            }
+            bool dup = false;
+            for (uint32_t x = 0; x < w; x++)
+                dup |= dst[x] == id;
+            if (!dup)
+                dst[w++] = id;
+        }
+    }
+    uint32_t *d = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), clean.size() * sizeof(uint32_t));
+    if (e != hipSuccess)
+        return hip_fail(e, "hipMalloc(adjacency)");
+    e = hipMemcpy(d, clean.data(), clean.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "upload of adjacency");
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    if (ix->d_adj) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ix->d_adj);
+    }
+    if (ix->graph_exec) {
+        (void)hipGraphExecDestroy(ix->graph_exec);
+        ix->graph_exec = nullptr;
+    }
+    if (ix->graph) {
+        (void)hipGraphDestroy(ix->graph);
+        ix->graph = nullptr;
+    }
+    ix->d_adj = d;
+    ix->r_pad = r_pad;
+    ix->ep = ep;
+    ix->has_graph = true;
+    return SS_OK;
+}
+
+int ss_get_graph(const ss_index *ix, uint32_t *adj_out, uint32_t *r_pad, uint32_t *ep)
+{
+    if (!ix || !ix->has_graph) {
+        set_error("ss_get_graph: no graph installed");
+        return SS_ESTATE;
+    }
+    if (r_pad)
+        *r_pad = ix->r_pad;
+    if (ep)
+        *ep = ix->ep;
+    if (adj_out) {
+        hipError_t e = hipMemcpy(adj_out, ix->d_adj, (size_t)ix->num * ix->r_pad * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(e, "download of adjacency");
+    }
+    return SS_OK;
+}
+
+int ss_load_graph(ss_index *ix, const char *path)
+{
+    if (!ix || !path) {
+        set_error("ss_load_graph: NULL argument");
+        return SS_EINVAL;
+    }
+    uint32_t num = 0, maxdeg = 0, ep = 0;
+    int rc = ss_nsg_header(path, &num, &maxdeg, &ep);
+    if (rc)
+        return rc;
+    if (num != ix->num) {
+        set_error("ss_load_graph: graph has " + std::to_string(num) + " nodes, index has " + std::to_string(ix->num));
+        return SS_EIO;
+    }
+    if (maxdeg == 0 || maxdeg > SS_MAX_RPAD) {
+        set_error("ss_load_graph: largest degree " + std::to_string(maxdeg) + " outside 1..64");
+        return SS_EIO;
+    }
+    std::vector<uint32_t> adj((size_t)num * maxdeg);
+    if ((rc = ss_nsg_read(path, adj.data(), num, maxdeg)))
+        return rc;
+    return ss_set_graph(ix, adj.data(), maxdeg, ep);
+}
+
+// ---------------------------------------------------------------------------
+// Launch shape.  The visited hash gets about 64 slots per pool entry (room
+// for the candidates of the expansions a pool of L needs), the waves per
+// block and the grid fill the LDS of every CU.
+// ---------------------------------------------------------------------------
+struct Shape {
+    uint32_t wpb, slots, blocks;
+    size_t lds_wave;
+};
+
+static uint32_t next_pow2(uint32_t x)
+{
+    uint32_t p = 1;
+    while (p < x)
+        p <<= 1;
+    return p;
+}
+
+static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const ss_search_params *p, Shape *s)
+{
+    uint32_t slots = p && p->visited_slots ? p->visited_slots : std::min<uint32_t>(8192, std::max<uint32_t>(1024, next_pow2(l_max * 64)));
+    if (slots < 256 || slots > 32768 || (slots & (slots - 1))) {
+        set_error("visited_slots must be a power of two in 256..32768");
+        return SS_EINVAL;
+    }
+    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots);
+    const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
+    const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
+    if (waves_cu == 0) {
+        set_error("search state does not fit in LDS: lower L or visited_slots");
+        return SS_EINVAL;
+    }
+    uint32_t wpb = p && p->waves_per_block ? p->waves_per_block : std::min<uint32_t>(waves_cu, 4);
+    if (wpb == 0 || wpb > 8 || wpb > waves_cu) {
+        set_error("waves_per_block must be 1..8 and fit in LDS");
+        return SS_EINVAL;
+    }
+    const uint32_t blocks_cu = waves_cu / wpb;
+    uint32_t blocks = p && p->blocks ? p->blocks : (uint32_t)ix->num_cus * blocks_cu;
+    const uint32_t need = (nq + wpb - 1) / wpb;
+    blocks = std::max<uint32_t>(1, std::min(blocks, need));
+    s->wpb = wpb;
+    s->slots = slots;
+    s->blocks = blocks;
+    s->lds_wave = per_wave;
+    return SS_OK;
+}
+
+static uint32_t log2u(uint32_t x)
+{
+    uint32_t r = 0;
+    while ((1u << r) < x)
+        r++;
+    return r;
+}
+
+static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
+                      uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
+                      uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st)
+{
+    if (!ix->has_graph) {
+        set_error("search: no graph loaded");
+        return SS_ESTATE;
+    }
+    if (nq == 0)
+        return SS_OK;
+    if (!d_queries || !d_out_ids) {
+        set_error("search: NULL buffer");
+        return SS_EINVAL;
+    }
+    if (k_max == 0 || l_max == 0 || k_max > l_max || l_max > SS_MAX_L || k_max > k_stride) {
+        set_error("search: need 1 <= K <= L <= 512 and K <= k_stride");  // tests/search.cpp:54-61
+        return SS_EINVAL;
+    }
+    const uint32_t mode = p ? p->mode : SS_MODE_BATCHED;
+    if (mode != SS_MODE_BATCHED && mode != SS_MODE_PER_QUERY) {
+        set_error("search: unknown mode");
+        return SS_EINVAL;
+    }
+    Shape sh;
+    int rc = choose_shape(ix, l_max, mode == SS_MODE_PER_QUERY ? 1 : nq, p, &sh);
+    if (rc)
+        return rc;
+    ss::SearchArgs a;
+    std::memset(&a, 0, sizeof(a));  // compared bytewise by the graph cache
+    a.base = ix->d_base;
+    a.adj = ix->d_adj;
+    a.queries = d_queries;
+    a.Ks = d_K;
+    a.Ls = d_L;
+    a.k_default = k_max;
+    a.l_default = l_max;
+    a.nq = nq;
+    a.dim_pad = ix->dim_pad;
+    a.r_pad = ix->r_pad;
+    a.ep = ix->ep;
+    a.l_max = l_max;
+    a.hash_slots = sh.slots;
+    a.hash_shift = 32u - log2u(sh.slots);
+    a.k_stride = k_stride;
+    a.q_first = 0;
+    a.out_ids = d_out_ids;
+    a.out_dists = d_out_dists;
+    a.stats = d_stats;
+    hipError_t e;
+    if (mode == SS_MODE_PER_QUERY) {
+        a.queue = nullptr;
+        a.nq = 1;
+        for (uint32_t q = 0; q < nq; q++) {
+            a.q_first = q;
+            if ((e = ss::launch_search(ix->metric, a, 1, 1, st)) != hipSuccess)
+                return hip_fail(e, "search launch");
+        }
+        return SS_OK;
+    }
+    a.queue = ix->d_queue;
+    const bool use_graph = p && p->use_graph;
+    if (use_graph) {
+        const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
+                          ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
+        if (!same) {
+            if (ix->graph_exec) {
+                (void)hipGraphExecDestroy(ix->graph_exec);
+                ix->graph_exec = nullptr;
+            }
+            if (ix->graph) {
+                (void)hipGraphDestroy(ix->graph);
+                ix->graph = nullptr;
+            }
+            if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return hip_fail(e, "hipStreamBeginCapture");
+            hipError_t e1 = ss::launch_queue_reset(ix->d_queue, st);
+            hipError_t e2 = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(st, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e), "search graph capture");
+            }
+            hipGraphExec_t ge = nullptr;
+            if ((e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return hip_fail(e, "hipGraphInstantiate");
+            }
+            ix->graph = g;
+            ix->graph_exec = ge;
+            ix->graph_args = a;
+            ix->graph_blocks = sh.blocks;
+            ix->graph_wpb = sh.wpb;
+            ix->graph_stream = st;
+        }
+        if ((e = hipGraphLaunch(ix->graph_exec, st)) != hipSuccess)
+            return hip_fail(e, "hipGraphLaunch");
+        return SS_OK;
+    }
+    if ((e = ss::launch_queue_reset(ix->d_queue, st)) != hipSuccess)
+        return hip_fail(e, "queue reset");
+    if ((e = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st)) != hipSuccess)
+        return hip_fail(e, "search launch");
+    return SS_OK;
+}
+
+int ss_search_device(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
+                     uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *params,
+                     uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, void *stream)
+{
+    if (!ix) {
+        set_error("ss_search_device: NULL index");
+        return SS_EINVAL;
+    }
+    return run_search(ix, d_queries, nq, d_K, d_L, k_max, l_max, k_stride, params, d_out_ids, d_out_dists, d_stats,
+                      reinterpret_cast<hipStream_t>(stream));
+}
+
+int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K, const uint32_t *L,
+              uint32_t k_default, uint32_t l_default, uint32_t k_stride, const ss_search_params *params,
+              uint32_t *out_ids, float *out_dists, ss_query_stats *stats)
+{
+    if (!ix) {
+        set_error("ss_search: NULL index");
+        return SS_EINVAL;
+    }
+    if (nq == 0)
+        return SS_OK;
+    if (!queries || !out_ids) {
+        set_error("ss_search: NULL buffer");
+        return SS_EINVAL;
+    }
+    uint32_t k_max = k_default, l_max = l_default;
+    if (K || L) {
+        k_max = K ? 0 : k_default;
+        l_max = L ? 0 : l_default;
+        for (uint32_t i = 0; i < nq; i++) {
+            const uint32_t ki = K ? K[i] : k_default;
+            const uint32_t li = L ? L[i] : l_default;
+            if (ki == 0 || li == 0 || ki > li || li > SS_MAX_L) {
+                set_error("ss_search: query " + std::to_string(i) + " needs 1 <= K <= L <= 512");
+                return SS_EINVAL;
+            }
+            k_max = std::max(k_max, ki);
+            l_max = std::max(l_max, li);
+        }
+    }
+    if (k_max > k_stride) {
+        set_error("ss_search: K larger than k_stride");
+        return SS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    hipError_t e;
+    if ((e = ix->q.reserve((size_t)nq * ix->dim_pad)) != hipSuccess || (e = ix->kl.reserve((size_t)nq * 2)) != hipSuccess ||
+        (e = ix->ids.reserve((size_t)nq * k_stride)) != hipSuccess ||
+        (e = ix->dists.reserve((size_t)nq * k_stride)) != hipSuccess || (e = ix->stats.reserve(nq)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(search buffers)");
+    hipStream_t st = ix->stream;
+    if (ix->dim_pad == ix->dim) {
+        e = hipMemcpyAsync(ix->q.ptr, queries, (size_t)nq * ix->dim * sizeof(float), hipMemcpyHostToDevice, st);
+    } else {
+        std::vector<float> padded((size_t)nq * ix->dim_pad, 0.0f);
+        for (uint32_t i = 0; i < nq; i++)
+            std::memcpy(&padded[(size_t)i * ix->dim_pad], queries + (size_t)i * ix->dim, ix->dim * sizeof(float));
+        e = hipMemcpy(ix->q.ptr, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "upload of queries");
+    uint32_t *dK = nullptr, *dL = nullptr;
+    if (K) {
+        dK = ix->kl.ptr;
+        if ((e = hipMemcpyAsync(dK, K, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st)) != hipSuccess)
+            return hip_fail(e, "upload of K");
+    }
+    if (L) {
+        dL = ix->kl.ptr + nq;
+        if ((e = hipMemcpyAsync(dL, L, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st)) != hipSuccess)
+            return hip_fail(e, "upload of L");
+    }
+    int rc = run_search(ix, ix->q.ptr, nq, dK, dL, k_max, l_max, k_stride, params, ix->ids.ptr, ix->dists.ptr,
+                        ix->stats.ptr, st);
+    if (rc)
+        return rc;
+    if ((e = hipMemcpyAsync(out_ids, ix->ids.ptr, (size_t)nq * k_stride * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            st)) != hipSuccess)
+        return hip_fail(e, "download of ids");
+    if (out_dists &&
+        (e = hipMemcpyAsync(out_dists, ix->dists.ptr, (size_t)nq * k_stride * sizeof(float), hipMemcpyDeviceToHost,
+                            st)) != hipSuccess)
+        return hip_fail(e, "download of distances");
+    if (stats && (e = hipMemcpyAsync(stats, ix->stats.ptr, (size_t)nq * sizeof(ss_query_stats),
+                                     hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "download of stats");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "search");
+    return SS_OK;
+}
+
+int ss_bruteforce_device(ss_index *ix, const float *d_queries, uint32_t nq, uint32_t k, int exclude_self,
+                         uint32_t query_base_offset, uint32_t *d_out_ids, float *d_out_dists, void *stream)
+{
+    if (!ix || !d_queries || !d_out_ids || !d_out_dists) {
+        set_error("ss_bruteforce_device: NULL argument");
+        return SS_EINVAL;
+    }
+    if (k == 0 || k > ss::bruteforce_kmax()) {
+        set_error("ss_bruteforce: k must be 1..128");
+        return SS_EINVAL;
+    }
+    if (nq == 0)
+        return SS_OK;
+    if (ss::bruteforce_lds_bytes(ix->dim_pad) > (ix->lds_per_cu ? ix->lds_per_cu : 65536)) {
+        set_error("ss_bruteforce: dimension too large for the brute-force tile");
+        return SS_EINVAL;
+    }
+    ss::BruteArgs a{};
+    a.base = ix->d_base;
+    a.queries = d_queries;
+    a.num = ix->num;
+    a.nq = nq;
+    a.dim_pad = ix->dim_pad;
+    a.k = k;
+    a.exclude_self = exclude_self;
+    a.self_offset = query_base_offset;
+    a.out_ids = d_out_ids;
+    a.out_dists = d_out_dists;
+    hipError_t e = ss::launch_bruteforce(ix->metric, a, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess)
+        return hip_fail(e, "brute-force launch");
+    return SS_OK;
+}
+
+static int bruteforce_host(ss_index *ix, const float *d_queries, uint32_t nq, uint32_t k, int self,
+                           uint32_t *out_ids, float *out_dists)
+{
+    hipError_t e;
+    // Chunks keep the result buffers small for the k-NN of a large base.
+    const uint32_t chunk = 65536;
+    if ((e = ix->ids.reserve((size_t)std::min(nq, chunk) * k)) != hipSuccess ||
+        (e = ix->dists.reserve((size_t)std::min(nq, chunk) * k)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(brute-force buffers)");
+    for (uint32_t q0 = 0; q0 < nq; q0 += chunk) {
+        const uint32_t n = std::min(chunk, nq - q0);
+        int rc = ss_bruteforce_device(ix, d_queries + (size_t)q0 * ix->dim_pad, n, k, self, q0, ix->ids.ptr,
+                                      ix->dists.ptr, ix->stream);
+        if (rc)
+            return rc;
+        if ((e = hipMemcpyAsync(out_ids + (size_t)q0 * k, ix->ids.ptr, (size_t)n * k * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, ix->stream)) != hipSuccess)
+            return hip_fail(e, "download of brute-force ids");
+        if (out_dists && (e = hipMemcpyAsync(out_dists + (size_t)q0 * k, ix->dists.ptr, (size_t)n * k * sizeof(float),
+                                             hipMemcpyDeviceToHost, ix->stream)) != hipSuccess)
+            return hip_fail(e, "download of brute-force distances");
+        if ((e = hipStreamSynchronize(ix->stream)) != hipSuccess)
+            return hip_fail(e, "brute force");
+    }
+    return SS_OK;
+}
+
+int ss_bruteforce(ss_index *ix, const float *queries, uint32_t nq, uint32_t k, const uint32_t *exclude,
+                  uint32_t *out_ids, float *out_dists)
+{
+    if (!ix || !queries || !out_ids) {
+        set_error("ss_bruteforce: NULL argument");
+        return SS_EINVAL;
+    }
+    if (exclude) {
+        // Per-query exclusions are served by running each excluded query
+        // with one extra neighbour and dropping the excluded id.
+        std::vector<uint32_t> ids((size_t)nq * (k + 1));
+        std::vector<float> ds((size_t)nq * (k + 1));
+        int rc = ss_bruteforce(ix, queries, nq, k + 1, nullptr, ids.data(), ds.data());
+        if (rc)
+            return rc;
+        for (uint32_t q = 0; q < nq; q++) {
+            uint32_t w = 0;
+            for (uint32_t j = 0; j < k + 1 && w < k; j++) {
+                if (ids[(size_t)q * (k + 1) + j] == exclude[q])
+                    continue;
+                out_ids[(size_t)q * k + w] = ids[(size_t)q * (k + 1) + j];
+                if (out_dists)
+                    out_dists[(size_t)q * k + w] = ds[(size_t)q * (k + 1) + j];
+                w++;
+            }
+        }
+        return SS_OK;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    hipError_t e;
+    if ((e = ix->q.reserve((size_t)nq * ix->dim_pad)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(queries)");
+    std::vector<float> padded((size_t)nq * ix->dim_pad, 0.0f);
+    for (uint32_t i = 0; i < nq; i++)
+        std::memcpy(&padded[(size_t)i * ix->dim_pad], queries + (size_t)i * ix->dim, ix->dim * sizeof(float));
+    if ((e = hipMemcpy(ix->q.ptr, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "upload of queries");
+    return bruteforce_host(ix, ix->q.ptr, nq, k, 0, out_ids, out_dists);
+}
+
+int ss_knn_graph(ss_index *ix, uint32_t k, uint32_t *out_ids, float *out_dists)
+{
+    if (!ix || !out_ids) {
+        set_error("ss_knn_graph: NULL argument");
+        return SS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    return bruteforce_host(ix, ix->d_base, ix->num, k, 1, out_ids, out_dists);
+}
+
+}  // extern "C"

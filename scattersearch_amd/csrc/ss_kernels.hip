@@ -1,0 +1,807 @@
+// Device code of the graph-ANN query path for MI355X (gfx950, wave64).
+//
+// search_kernel   best-first search over the padded adjacency, one wavefront
+//                 per query, pool + visited hash + query in LDS
+//                 (ANNSearch::SearchArraySimulation, src/annsearch.cpp:182-259)
+// bf_kernel       exact top-k by (distance, id) for ground truth and k-NN
+//                 graph construction
+//
+// Distance arithmetic.  Both kernels compute the reference's AVX distances
+// (src/distance.cpp:19-56, 115-149) bit for bit as its own -Ofast build
+// evaluates them: eight partial sums s[j]; an optional leading tail block
+// s[j] = x[j]*y[j]; then per 16-float block
+//     s[j] = s[j] + fma(x0[j], y0[j], x1[j]*y1[j]);
+// and the pairwise sum ((s0+s1)+(s2+s3))+((s4+s5)+(s6+s7)).  The file is
+// compiled with -ffp-contract=off so nothing else gets fused.
+//
+// In the search a team of two lanes computes one distance: lane h of the team
+// owns partials 4h..4h+3 and reads them with 16-byte loads (floats
+// 16b+4h..16b+4h+3 and 16b+8+4h..16b+8+4h+3 of every block b), so one
+// wavefront gathers 32 neighbour vectors at once with coalesced dwordx4 loads.
+
+#include "ss_internal.h"
+
+namespace ss {
+namespace {
+
+constexpr int kMaxChunks = SS_MAX_L / 64;
+
+__device__ __forceinline__ void wave_sync()
+{
+    // LDS traffic of one wavefront executes in program order; this keeps the
+    // compiler from moving LDS accesses of one lane across accesses of
+    // another lane to the same words.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ float readlane_f(float v, uint32_t lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// ---------------------------------------------------------------------------
+// Team distance (two lanes per vector).  NBLK < 0 selects the runtime-size
+// form; otherwise NBLK full 16-float blocks and TAILI (0/1) leading tail
+// block are compile-time constants.
+// ---------------------------------------------------------------------------
+template <int NBLK, int TAILI, int METRIC>
+__device__ __forceinline__ float team_dist(const float *__restrict__ row, const float *q, int h,
+                                           int nblk_rt, int tail_rt)
+{
+    const int nblk = NBLK >= 0 ? NBLK : nblk_rt;
+    const bool tail = NBLK >= 0 ? (TAILI != 0) : (tail_rt != 0);
+    const float4 *r4 = reinterpret_cast<const float4 *>(row);
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (tail) {
+        const float4 x = r4[4 * nblk + h];
+        const float4 y = q4[4 * nblk + h];
+        if (METRIC == SS_METRIC_L2) {
+            const float d0 = y.x - x.x, d1 = y.y - x.y, d2 = y.z - x.z, d3 = y.w - x.w;
+            s0 = d0 * d0;
+            s1 = d1 * d1;
+            s2 = d2 * d2;
+            s3 = d3 * d3;
+        } else {
+            s0 = y.x * x.x;
+            s1 = y.y * x.y;
+            s2 = y.z * x.z;
+            s3 = y.w * x.w;
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < nblk; b++) {
+        const float4 xa = r4[4 * b + h];
+        const float4 xb = r4[4 * b + 2 + h];
+        const float4 qa = q4[4 * b + h];
+        const float4 qb = q4[4 * b + 2 + h];
+        if (METRIC == SS_METRIC_L2) {
+            const float a0 = qa.x - xa.x, a1 = qa.y - xa.y, a2 = qa.z - xa.z, a3 = qa.w - xa.w;
+            const float b0 = qb.x - xb.x, b1 = qb.y - xb.y, b2 = qb.z - xb.z, b3 = qb.w - xb.w;
+            s0 = s0 + __builtin_fmaf(a0, a0, b0 * b0);
+            s1 = s1 + __builtin_fmaf(a1, a1, b1 * b1);
+            s2 = s2 + __builtin_fmaf(a2, a2, b2 * b2);
+            s3 = s3 + __builtin_fmaf(a3, a3, b3 * b3);
+        } else {
+            s0 = s0 + __builtin_fmaf(qa.x, xa.x, qb.x * xb.x);
+            s1 = s1 + __builtin_fmaf(qa.y, xa.y, qb.y * xb.y);
+            s2 = s2 + __builtin_fmaf(qa.z, xa.z, qb.z * xb.z);
+            s3 = s3 + __builtin_fmaf(qa.w, xa.w, qb.w * xb.w);
+        }
+    }
+    const float part = (s0 + s1) + (s2 + s3);
+    const float other = __shfl_xor(part, 1, 64);
+    const float r = part + other;  // lane 0 of the team: low + high; lane 1: high + low (same bits)
+    return METRIC == SS_METRIC_INNER_PRODUCT ? -r : r;
+}
+
+// Whole-vector distance in one lane (brute force).  q is the query, row the
+// base vector; same arithmetic as team_dist.
+template <int NBLK, int TAILI, int METRIC>
+__device__ __forceinline__ float lane_dist(const float *__restrict__ row, const float *q, int nblk_rt,
+                                           int tail_rt)
+{
+    const int nblk = NBLK >= 0 ? NBLK : nblk_rt;
+    const bool tail = NBLK >= 0 ? (TAILI != 0) : (tail_rt != 0);
+    const float4 *r4 = reinterpret_cast<const float4 *>(row);
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    float s[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        s[j] = 0.f;
+    if (tail) {
+        const float4 x0 = r4[4 * nblk], x1 = r4[4 * nblk + 1];
+        const float4 y0 = q4[4 * nblk], y1 = q4[4 * nblk + 1];
+        const float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        const float yv[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (METRIC == SS_METRIC_L2) {
+                const float d = yv[j] - xv[j];
+                s[j] = d * d;
+            } else {
+                s[j] = yv[j] * xv[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < nblk; b++) {
+        const float4 xa0 = r4[4 * b], xa1 = r4[4 * b + 1], xb0 = r4[4 * b + 2], xb1 = r4[4 * b + 3];
+        const float4 qa0 = q4[4 * b], qa1 = q4[4 * b + 1], qb0 = q4[4 * b + 2], qb1 = q4[4 * b + 3];
+        const float xa[8] = {xa0.x, xa0.y, xa0.z, xa0.w, xa1.x, xa1.y, xa1.z, xa1.w};
+        const float xb[8] = {xb0.x, xb0.y, xb0.z, xb0.w, xb1.x, xb1.y, xb1.z, xb1.w};
+        const float qa[8] = {qa0.x, qa0.y, qa0.z, qa0.w, qa1.x, qa1.y, qa1.z, qa1.w};
+        const float qb[8] = {qb0.x, qb0.y, qb0.z, qb0.w, qb1.x, qb1.y, qb1.z, qb1.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (METRIC == SS_METRIC_L2) {
+                const float a = qa[j] - xa[j];
+                const float c = qb[j] - xb[j];
+                s[j] = s[j] + __builtin_fmaf(a, a, c * c);
+            } else {
+                s[j] = s[j] + __builtin_fmaf(qa[j], xa[j], qb[j] * xb[j]);
+            }
+        }
+    }
+    const float r = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    return METRIC == SS_METRIC_INNER_PRODUCT ? -r : r;
+}
+
+// ---------------------------------------------------------------------------
+// Visited set: open-addressed hash of node ids in LDS, linear probing.  A
+// lane that finds no free slot within kMaxProbe probes treats its id as
+// unvisited (and counts a lossy insert).  Forgetting a visited node never
+// changes the search result: its recomputed distance is either not below the
+// pool tail (the tail only decreases) or it is still in the pool, where the
+// merge drops the repeat; only the distance count grows.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool visit(uint32_t *tab, uint32_t shift, uint32_t mask, uint32_t id,
+                                      uint32_t &lossy)
+{
+    uint32_t s = (id * 0x9E3779B1u) >> shift;
+    for (uint32_t p = 0; p < kMaxProbe; p++) {
+        const uint32_t cur = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == id)
+            return false;
+        if (cur == kNone) {
+            const uint32_t old = atomicCAS(&tab[s], kNone, id);
+            if (old == kNone)
+                return true;
+            if (old == id)
+                return false;
+        }
+        s = (s + 1u) & mask;
+    }
+    lossy++;
+    return true;
+}
+
+struct WaveLds {
+    float *qry;
+    uint32_t *pool_id;
+    float *pool_d;
+    uint32_t *cand_id;
+    float *cand_d;
+    uint32_t *hash;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots)
+{
+    return align16(dim_pad * 4u) + align16(l_max * 4u) * 2 + 64 * 4 * 2 + align16(hash_slots * 4u);
+}
+
+__device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_max)
+{
+    WaveLds w;
+    w.qry = reinterpret_cast<float *>(p);
+    p += align16(dim_pad * 4u);
+    w.pool_id = reinterpret_cast<uint32_t *>(p);
+    p += align16(l_max * 4u);
+    w.pool_d = reinterpret_cast<float *>(p);
+    p += align16(l_max * 4u);
+    w.cand_id = reinterpret_cast<uint32_t *>(p);
+    p += 64 * 4;
+    w.cand_d = reinterpret_cast<float *>(p);
+    p += 64 * 4;
+    w.hash = reinterpret_cast<uint32_t *>(p);
+    return w;
+}
+
+// Distances of candidates cand_id[0..nc) into cand_d[0..nc), 32 per pass.
+template <int NBLK, int TAILI, int METRIC>
+__device__ __forceinline__ void candidate_distances(const SearchArgs &a, const WaveLds &w, uint32_t nc,
+                                                    int nblk_rt, int tail_rt)
+{
+    const uint32_t lane = lane_id();
+    const int h = lane & 1;
+    for (uint32_t c0 = 0; c0 < nc; c0 += 32) {
+        const uint32_t c = c0 + (lane >> 1);
+        if (c < nc) {
+            const uint32_t id = w.cand_id[c];
+            const float d = team_dist<NBLK, TAILI, METRIC>(a.base + (size_t)id * a.dim_pad, w.qry, h,
+                                                           nblk_rt, tail_rt);
+            if (h == 0)
+                w.cand_d[c] = d;
+        }
+    }
+}
+
+// One query: src/annsearch.cpp:182-259 as a sequence of whole-row
+// expansions.  For the expansion of pool entry k with pool P (size t, tail
+// distance tau = P[t-1].dist), let A be the unvisited neighbours with
+// distance < tau in row order.  The reference's per-neighbour loop (tail
+// filter :238, InsertIntoPool :241 placing an entry after equal distances,
+// t grows to L :242-243) leaves exactly the first min(L, t+|A|) entries of
+// the stable merge of P and A (P first on equal distances, A in row order),
+// and its jump target nk (:244-252) is the merged position of the first
+// smallest element of A.  Every entry before the cursor is explored, so the
+// next node is the first unexplored pool entry.
+template <int NBLK, int TAILI, int METRIC>
+__device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
+{
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    const uint32_t K = uniform(a.Ks ? a.Ks[q] : a.k_default);
+    const uint32_t L = uniform(a.Ls ? a.Ls[q] : a.l_default);
+    uint32_t *out_ids = a.out_ids + (size_t)q * a.k_stride;
+    float *out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
+
+    if (K == 0 || L == 0 || K > L || L > a.l_max || K > a.k_stride) {
+        // Rejected by the host already; fill the row so nothing is stale.
+        for (uint32_t i = lane; i < a.k_stride; i += 64) {
+            out_ids[i] = kNone;
+            if (out_d)
+                out_d[i] = __builtin_inff();
+        }
+        if (a.stats && lane == 0)
+            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu};
+        return;
+    }
+
+    // Query vector and an empty visited hash.
+    const float4 *qsrc = reinterpret_cast<const float4 *>(a.queries + (size_t)q * a.dim_pad);
+    float4 *qdst = reinterpret_cast<float4 *>(w.qry);
+    for (uint32_t i = lane; i < a.dim_pad / 4; i += 64)
+        qdst[i] = qsrc[i];
+    uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
+    for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
+        h4[i] = make_uint4(kNone, kNone, kNone, kNone);
+    const uint32_t hmask = a.hash_slots - 1u;
+    uint32_t lossy = 0;
+    wave_sync();
+
+    // Seed pool: the first min(L, deg(ep)) neighbours of the entry point,
+    // marked visited (ep itself is neither pooled nor marked), :185-215.
+    uint32_t nb = lane < a.r_pad ? a.adj[(size_t)a.ep * a.r_pad + lane] : kNone;
+    const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
+    uint32_t t = deg < L ? deg : L;
+    if (lane < t) {
+        visit(w.hash, a.hash_shift, hmask, nb, lossy);
+        w.cand_id[lane] = nb;
+    }
+    wave_sync();
+    candidate_distances<NBLK, TAILI, METRIC>(a, w, t, nblk_rt, tail_rt);
+    wave_sync();
+    {
+        const float di = lane < t ? w.cand_d[lane] : __builtin_inff();
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < t; j++) {
+            const float dj = readlane_f(di, j);
+            rank += (dj < di || (dj == di && j < lane)) ? 1u : 0u;
+        }
+        wave_sync();
+        if (lane < t) {
+            w.pool_id[rank] = nb;
+            w.pool_d[rank] = di;
+        }
+    }
+    uint32_t ndc = t, hops = 0;
+    wave_sync();
+
+    uint32_t start = 0;
+    while (true) {
+        // Next node: first unexplored entry at or after the cursor.
+        int found = -1;
+        for (uint32_t c0 = start & ~63u; c0 < t; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            const bool un = i >= start && i < t && !(w.pool_id[i] & kExplored);
+            const uint64_t m = ballot(un);
+            if (m) {
+                found = (int)(c0 + (uint32_t)__ffsll((unsigned long long)m) - 1u);
+                break;
+            }
+        }
+        if (found < 0)
+            break;
+        const uint32_t k = uniform((uint32_t)found);
+        const uint32_t n = uniform(w.pool_id[k]);
+        const float tail = readlane_f(w.pool_d[t - 1], 0);
+        wave_sync();
+        if (lane == 0)
+            w.pool_id[k] = n | kExplored;
+        hops++;
+
+        // Neighbour row (one coalesced load), visited test, compaction.
+        nb = lane < a.r_pad ? a.adj[(size_t)n * a.r_pad + lane] : kNone;
+        const bool fresh = nb != kNone && visit(w.hash, a.hash_shift, hmask, nb, lossy);
+        const uint64_t fm = ballot(fresh);
+        const uint32_t nc = (uint32_t)__popcll(fm);
+        if (fresh)
+            w.cand_id[lanes_below(fm)] = nb;
+        wave_sync();
+        candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
+        ndc += nc;
+        wave_sync();
+
+        float cd = __builtin_inff();
+        uint32_t cid = kNone;
+        if (lane < nc) {
+            cd = w.cand_d[lane];
+            cid = w.cand_id[lane];
+        }
+        bool inA = lane < nc && cd < tail;  // strict tail filter, :238
+
+        const uint32_t nch = (t + 63u) >> 6;
+        uint32_t pid[kMaxChunks];
+        float pdv[kMaxChunks];
+        uint32_t cnt[kMaxChunks];
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; c++) {
+            pid[c] = kNone;
+            pdv[c] = __builtin_inff();
+            cnt[c] = 0;
+            const uint32_t i = (uint32_t)c * 64u + lane;
+            if ((uint32_t)c < nch && i < t) {
+                pid[c] = w.pool_id[i];
+                pdv[c] = w.pool_d[i];
+            }
+        }
+
+        // After a lossy insert a pool member can come back as a candidate
+        // (same id, same distance): drop it.
+        if (ballot(lossy != 0)) {
+            uint64_t it = ballot(inA);
+            while (it) {
+                const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
+                it &= it - 1ull;
+                const uint32_t x = readlane_u(cid, j);
+                bool hit = false;
+#pragma unroll
+                for (int c = 0; c < kMaxChunks; c++)
+                    hit |= (uint32_t)c < nch && (pid[c] & ~kExplored) == x && pid[c] != kNone;
+                if (ballot(hit) && lane == j)
+                    inA = false;
+            }
+        }
+
+        const uint64_t am = ballot(inA);
+        if (am == 0) {  // nothing admitted: nk = L, continue at k+1 (:249-252)
+            start = k + 1;
+            continue;
+        }
+        const uint32_t na = (uint32_t)__popcll(am);
+
+        // Merge ranks.  Pool entry i moves to i + #{a in A: a < p};
+        // candidate a goes to rankA(a) + #{p in P: p <= a}.
+        uint32_t rankA = 0, posA = 0;
+        uint64_t it = am;
+        while (it) {
+            const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
+            it &= it - 1ull;
+            const float dj = readlane_f(cd, j);
+            rankA += (inA && (dj < cd || (dj == cd && j < lane))) ? 1u : 0u;
+            uint32_t le = 0;
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; c++) {
+                if ((uint32_t)c < nch) {
+                    cnt[c] += dj < pdv[c] ? 1u : 0u;
+                    const uint32_t i = (uint32_t)c * 64u + lane;
+                    le += (uint32_t)__popcll(ballot(i < t && pdv[c] <= dj));
+                }
+            }
+            if (lane == j)
+                posA = le;
+        }
+        posA += rankA;
+        const uint64_t mm = ballot(inA && rankA == 0);
+        const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)mm) - 1u;
+        const uint32_t nk = readlane_u(posA, jmin);
+
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; c++) {
+            const uint32_t i = (uint32_t)c * 64u + lane;
+            if ((uint32_t)c < nch && i < t) {
+                const uint32_t np = i + cnt[c];
+                if (np < L) {
+                    w.pool_id[np] = pid[c];
+                    w.pool_d[np] = pdv[c];
+                }
+            }
+        }
+        if (inA && posA < L) {
+            w.pool_id[posA] = cid;
+            w.pool_d[posA] = cd;
+        }
+        t = t + na < L ? t + na : L;
+        start = nk <= k ? nk : k + 1;
+        wave_sync();
+    }
+
+    // indices[i] = retset[i].id, i < K (:254-257); distances as the
+    // pipeline form returns them (:319-323).
+    for (uint32_t i = lane; i < a.k_stride; i += 64) {
+        const bool ok = i < K && i < t;
+        out_ids[i] = ok ? (w.pool_id[i] & ~kExplored) : kNone;
+        if (out_d)
+            out_d[i] = ok ? w.pool_d[i] : __builtin_inff();
+    }
+    uint32_t lsum = lossy;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        lsum += __shfl_xor(lsum, off, 64);
+    if (a.stats && lane == 0)
+        a.stats[q] = ss_query_stats{ndc, hops, lsum, 0u};
+    wave_sync();
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t waves = blockDim.x >> 6;
+    const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots);
+    const WaveLds w = carve(smem + per_wave * wave, a.dim_pad, a.l_max);
+    if (a.queue) {
+        while (true) {
+            uint32_t q = 0;
+            if (lane_id() == 0)
+                q = atomicAdd(a.queue, 1u);
+            q = uniform(q);
+            if (q >= a.nq)
+                break;
+            search_one<NBLK, TAILI, METRIC>(a, w, a.q_first + q);
+        }
+    } else {
+        const uint32_t q = blockIdx.x * waves + wave;
+        if (q < a.nq)
+            search_one<NBLK, TAILI, METRIC>(a, w, a.q_first + q);
+    }
+}
+
+__global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
+
+// ---------------------------------------------------------------------------
+// Brute force: a block holds kBfQ queries in LDS and streams the base in
+// tiles of 256 points, one point per lane.  Distances below a query's current
+// k-th (distance, id) go to a per-query candidate list; after each tile one
+// wavefront per query merges the candidates into the sorted top-k.
+// ---------------------------------------------------------------------------
+constexpr int kBfQ = 16;
+constexpr int kBfT = 256;
+constexpr int kBfKMax = 128;
+
+__device__ __forceinline__ bool pair_less(float da, uint32_t ia, float db, uint32_t ib)
+{
+    return da < db || (da == db && ia < ib);
+}
+
+struct BfLds {
+    float *qry;       // [kBfQ][dim_pad]
+    float *top_d;     // [kBfQ][kBfKMax]
+    uint32_t *top_i;  // [kBfQ][kBfKMax]
+    float *cand_d;    // [kBfQ][kBfT]
+    uint32_t *cand_i; // [kBfQ][kBfT]
+    uint32_t *cand_n; // [kBfQ]
+    uint32_t *top_n;  // [kBfQ]
+    float *thr_d;     // [kBfQ]
+    uint32_t *thr_i;  // [kBfQ]
+};
+
+__host__ __device__ inline size_t bf_lds_bytes(uint32_t dim_pad)
+{
+    return align16((size_t)kBfQ * dim_pad * 4) + (size_t)kBfQ * kBfKMax * 8 + (size_t)kBfQ * kBfT * 8 +
+           (size_t)kBfQ * 16;
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(kBfT) void bf_kernel(BruteArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *p = smem;
+    BfLds s;
+    s.qry = reinterpret_cast<float *>(p);
+    p += align16((size_t)kBfQ * a.dim_pad * 4);
+    s.top_d = reinterpret_cast<float *>(p);
+    p += kBfQ * kBfKMax * 4;
+    s.top_i = reinterpret_cast<uint32_t *>(p);
+    p += kBfQ * kBfKMax * 4;
+    s.cand_d = reinterpret_cast<float *>(p);
+    p += kBfQ * kBfT * 4;
+    s.cand_i = reinterpret_cast<uint32_t *>(p);
+    p += kBfQ * kBfT * 4;
+    s.cand_n = reinterpret_cast<uint32_t *>(p);
+    p += kBfQ * 4;
+    s.top_n = reinterpret_cast<uint32_t *>(p);
+    p += kBfQ * 4;
+    s.thr_d = reinterpret_cast<float *>(p);
+    p += kBfQ * 4;
+    s.thr_i = reinterpret_cast<uint32_t *>(p);
+
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = tid >> 6;
+    const uint32_t q0 = blockIdx.x * kBfQ;
+    const uint32_t nql = a.nq - q0 < (uint32_t)kBfQ ? a.nq - q0 : (uint32_t)kBfQ;
+    const uint32_t k = a.k;
+
+    for (uint32_t i = tid; i < nql * a.dim_pad / 4; i += kBfT)
+        reinterpret_cast<float4 *>(s.qry)[i] =
+            reinterpret_cast<const float4 *>(a.queries + (size_t)q0 * a.dim_pad)[i];
+    if (tid < (uint32_t)kBfQ) {
+        s.cand_n[tid] = 0;
+        s.top_n[tid] = 0;
+        s.thr_d[tid] = __builtin_inff();
+        s.thr_i[tid] = kNone;
+    }
+    __syncthreads();
+
+    for (uint32_t tile = 0; tile < a.num; tile += kBfT) {
+        const uint32_t pidx = tile + tid;
+        if (pidx < a.num) {
+            const float *row = a.base + (size_t)pidx * a.dim_pad;
+            for (uint32_t ql = 0; ql < nql; ql++) {
+                if (a.exclude_self && pidx == a.self_offset + q0 + ql)
+                    continue;
+                const float d = lane_dist<NBLK, TAILI, METRIC>(row, s.qry + (size_t)ql * a.dim_pad,
+                                                               nblk_rt, tail_rt);
+                if (pair_less(d, pidx, s.thr_d[ql], s.thr_i[ql])) {
+                    const uint32_t slot = atomicAdd(&s.cand_n[ql], 1u);
+                    s.cand_d[ql * kBfT + slot] = d;
+                    s.cand_i[ql * kBfT + slot] = pidx;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t ql = wave; ql < nql; ql += kBfT / 64) {
+            const uint32_t c = s.cand_n[ql];
+            if (c == 0)
+                continue;
+            const uint32_t nt = s.top_n[ql];
+            const float *cdp = s.cand_d + ql * kBfT;
+            const uint32_t *cip = s.cand_i + ql * kBfT;
+            float *tdp = s.top_d + ql * kBfKMax;
+            uint32_t *tip = s.top_i + ql * kBfKMax;
+            float cdv[kBfT / 64];
+            uint32_t civ[kBfT / 64], cpos[kBfT / 64];
+            float tdv[kBfKMax / 64];
+            uint32_t tiv[kBfKMax / 64], tpos[kBfKMax / 64];
+#pragma unroll
+            for (int r = 0; r < kBfT / 64; r++) {
+                const uint32_t j = lane + 64u * r;
+                cdv[r] = j < c ? cdp[j] : __builtin_inff();
+                civ[r] = j < c ? cip[j] : kNone;
+                cpos[r] = 0;
+            }
+#pragma unroll
+            for (int r = 0; r < kBfKMax / 64; r++) {
+                const uint32_t i = lane + 64u * r;
+                tdv[r] = i < nt ? tdp[i] : __builtin_inff();
+                tiv[r] = i < nt ? tip[i] : kNone;
+                tpos[r] = i;
+            }
+            for (uint32_t x = 0; x < c; x++) {
+                const float dx = cdp[x];
+                const uint32_t ix = cip[x];
+#pragma unroll
+                for (int r = 0; r < kBfT / 64; r++)
+                    cpos[r] += pair_less(dx, ix, cdv[r], civ[r]) ? 1u : 0u;
+#pragma unroll
+                for (int r = 0; r < kBfKMax / 64; r++)
+                    tpos[r] += pair_less(dx, ix, tdv[r], tiv[r]) ? 1u : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < kBfT / 64; r++) {
+                uint32_t lo = 0, hi = nt;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pair_less(tdp[mid], tip[mid], cdv[r], civ[r]))
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                cpos[r] += lo;
+            }
+            wave_sync();
+#pragma unroll
+            for (int r = 0; r < kBfKMax / 64; r++) {
+                const uint32_t i = lane + 64u * r;
+                if (i < nt && tpos[r] < k) {
+                    tdp[tpos[r]] = tdv[r];
+                    tip[tpos[r]] = tiv[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kBfT / 64; r++) {
+                const uint32_t j = lane + 64u * r;
+                if (j < c && cpos[r] < k) {
+                    tdp[cpos[r]] = cdv[r];
+                    tip[cpos[r]] = civ[r];
+                }
+            }
+            wave_sync();
+            const uint32_t nn = nt + c < k ? nt + c : k;
+            if (lane == 0) {
+                s.top_n[ql] = nn;
+                s.cand_n[ql] = 0;
+                if (nn == k) {
+                    s.thr_d[ql] = tdp[k - 1];
+                    s.thr_i[ql] = tip[k - 1];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t ql = wave; ql < nql; ql += kBfT / 64) {
+        const uint32_t nt = s.top_n[ql];
+        for (uint32_t i = lane; i < k; i += 64) {
+            a.out_ids[(size_t)(q0 + ql) * k + i] = i < nt ? s.top_i[ql * kBfKMax + i] : kNone;
+            a.out_dists[(size_t)(q0 + ql) * k + i] = i < nt ? s.top_d[ql * kBfKMax + i] : __builtin_inff();
+        }
+    }
+}
+
+__global__ void pad_copy_kernel(const float *src, float *dst, uint32_t rows, uint32_t dim,
+                                uint32_t dim_pad)
+{
+    const size_t total = (size_t)rows * dim_pad;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / dim_pad, c = i % dim_pad;
+        dst[i] = c < dim ? src[r * dim + c] : 0.0f;
+    }
+}
+
+// Dimension dispatch: compile-time block counts for common sizes, the
+// runtime-size form for the rest.
+template <template <int, int, int> class F, int METRIC, typename... Args>
+hipError_t dispatch_dim(uint32_t dim_pad, Args &&...args)
+{
+    switch (dim_pad) {
+    case 96:
+        return F<6, 0, METRIC>::run(args...);
+    case 128:
+        return F<8, 0, METRIC>::run(args...);
+    case 200:
+        return F<12, 1, METRIC>::run(args...);
+    default:
+        return F<-1, 0, METRIC>::run(args...);
+    }
+}
+
+template <int NBLK, int TAILI, int METRIC>
+struct SearchLaunch {
+    static hipError_t run(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
+    {
+        const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots) * wpb;
+        hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+        return hipGetLastError();
+    }
+    static hipError_t attr(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
+};
+
+template <int NBLK, int TAILI, int METRIC>
+struct BfLaunch {
+    static hipError_t run(const BruteArgs &a, hipStream_t st)
+    {
+        const size_t lds = bf_lds_bytes(a.dim_pad);
+        const uint32_t blocks = (a.nq + kBfQ - 1) / kBfQ;
+        hipLaunchKernelGGL((bf_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(kBfT), lds, st, a);
+        return hipGetLastError();
+    }
+    static hipError_t attr(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(bf_kernel<NBLK, TAILI, METRIC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
+};
+
+template <template <int, int, int> class F, int METRIC>
+hipError_t attr_all(size_t max_lds)
+{
+    hipError_t e;
+    if ((e = F<6, 0, METRIC>::attr(max_lds)) != hipSuccess)
+        return e;
+    if ((e = F<8, 0, METRIC>::attr(max_lds)) != hipSuccess)
+        return e;
+    if ((e = F<12, 1, METRIC>::attr(max_lds)) != hipSuccess)
+        return e;
+    return F<-1, 0, METRIC>::attr(max_lds);
+}
+
+}  // namespace
+
+size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots)
+{
+    return wave_lds_bytes(dim_pad, l_max, hash_slots);
+}
+
+size_t bruteforce_lds_bytes(uint32_t dim_pad) { return bf_lds_bytes(dim_pad); }
+
+uint32_t bruteforce_kmax() { return kBfKMax; }
+
+hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint32_t wpb,
+                         hipStream_t stream)
+{
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<SearchLaunch, SS_METRIC_INNER_PRODUCT>(a.dim_pad, a, blocks, wpb, stream);
+    return dispatch_dim<SearchLaunch, SS_METRIC_L2>(a.dim_pad, a, blocks, wpb, stream);
+}
+
+hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream)
+{
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream)
+{
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<BfLaunch, SS_METRIC_INNER_PRODUCT>(a.dim_pad, a, stream);
+    return dispatch_dim<BfLaunch, SS_METRIC_L2>(a.dim_pad, a, stream);
+}
+
+hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t dim, uint32_t dim_pad,
+                           hipStream_t stream)
+{
+    hipLaunchKernelGGL(pad_copy_kernel, dim3(1024), dim3(256), 0, stream, src, dst, rows, dim, dim_pad);
+    return hipGetLastError();
+}
+
+hipError_t init_kernel_attributes(size_t max_lds)
+{
+    hipError_t e;
+    if ((e = attr_all<SearchLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<SearchLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<BfLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    return attr_all<BfLaunch, SS_METRIC_INNER_PRODUCT>(max_lds);
+}
+
+bool dim_supported(uint32_t dim_pad) { return dim_pad >= 8 && dim_pad % 8 == 0 && dim_pad <= 4096; }
+
+}  // namespace ss

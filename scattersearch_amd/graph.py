@@ -1,0 +1,99 @@
+"""NSG-style graph construction for the synthetic workloads.
+
+The reference only reads .nsg files made by the external NSG tool
+(src/annsearch.cpp:64-80).  This pipeline makes them:
+
+1. exact k-NN of every base point (GPU brute force, bit-exact with the CPU);
+2. entry point = the point nearest the centroid (NSG's navigating node);
+3. navigation candidates: the pool of a search for every point from the
+   entry point over the k-NN graph (NSG takes candidates from this search
+   path, which is what gives the graph its long-range edges between
+   clusters);
+4. occlusion prune of k-NN + navigation candidates to R (MRNG with the
+   relaxation factor alpha of DiskANN's RobustPrune), reverse edges, and a
+   pass that attaches every node not reachable from the entry point
+   (ss_build_graph in the C ABI);
+5. steps 3-4 again with the navigation search over the graph of step 4.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, fptr, lib, uptr
+
+
+def medoid(base):
+    base = np.ascontiguousarray(base, dtype=np.float32)
+    ep = ctypes.c_uint32()
+    check(lib.ss_medoid(fptr(base), base.shape[0], base.shape[1], ctypes.byref(ep)))
+    return ep.value
+
+
+def prune_and_connect(base, metric, knn, extra, R, ep, alpha=1.0, nthreads=0):
+    base = np.ascontiguousarray(base, dtype=np.float32)
+    knn = np.ascontiguousarray(knn, dtype=np.uint32)
+    ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.uint32)
+    adj = np.empty((base.shape[0], R), dtype=np.uint32)
+    fixed = ctypes.c_uint32()
+    check(lib.ss_build_graph(fptr(base), base.shape[0], base.shape[1], int(metric), uptr(knn),
+                             knn.shape[1], uptr(ex), 0 if ex is None else ex.shape[1], R, alpha, ep,
+                             nthreads, uptr(adj), ctypes.byref(fixed)))
+    return adj, fixed.value
+
+
+def reachable(adj, ep):
+    a = np.ascontiguousarray(adj, dtype=np.uint32)
+    r = ctypes.c_uint32()
+    check(lib.ss_graph_reachable(uptr(a), a.shape[0], a.shape[1], ep, ctypes.byref(r)))
+    return r.value
+
+
+def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.2, passes=2,
+                nthreads=0, log=None):
+    """knn_fn(k) -> [num][k] ids; search_fn(adj, ep, L) -> [num][L] pools of
+    a search for every base point over graph adj.  The first pass takes its
+    navigation candidates from a search over the k-NN graph; each further
+    pass searches the graph of the pass before (so the candidates follow the
+    long-range edges that pass created).  Returns (adj [num][R], ep, info)."""
+    import time
+    t0 = time.time()
+    knn = knn_fn(knn_k)
+    t1 = time.time()
+    ep = medoid(base)
+    info = dict(knn_s=t1 - t0, passes=[])
+    adj = knn
+    for p in range(max(1, passes)):
+        ta = time.time()
+        nav = search_fn(adj, ep, nav_L) if nav_L else None
+        tb = time.time()
+        adj, fixed = prune_and_connect(base, metric, knn, nav, R, ep, alpha, nthreads)
+        tc = time.time()
+        deg = float((adj != 0xFFFFFFFF).sum(1).mean())
+        info["passes"].append(dict(nav_s=tb - ta, prune_s=tc - tb, attached=fixed, mean_degree=deg))
+        if log:
+            log(f"graph pass {p}: nav {tb - ta:.1f}s prune {tc - tb:.1f}s attached {fixed} "
+                f"mean degree {deg:.1f}")
+    if log:
+        log(f"graph: knn {info['knn_s']:.1f}s total {time.time() - t0:.1f}s")
+    return adj, ep, info
+
+
+def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.2, passes=2, chunk=131072, nthreads=0,
+                    log=None):
+    """build_graph with the GPU providers of an Index over `base`."""
+
+    def knn_fn(k):
+        return index.knn_graph(k)[0]
+
+    def search_fn(knn, ep, L):
+        index.set_graph(knn, ep)
+        out = np.empty((base.shape[0], L), dtype=np.uint32)
+        for s in range(0, base.shape[0], chunk):
+            ids, _, _ = index.search(base[s:s + chunk], L, L)
+            out[s:s + chunk] = ids
+        return out
+
+    adj, ep, info = build_graph(base, index.metric, R, knn_k, nav_L, knn_fn, search_fn, alpha,
+                                passes, nthreads, log)
+    index.set_graph(adj, ep)
+    return adj, ep, info

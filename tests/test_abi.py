@@ -1,0 +1,67 @@
+"""The C-ABI library loads and exports every function include/scattersearch.h
+declares; entry points that need no GPU work without one."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_available
+
+HEADER = os.path.join(ROOT, "include", "scattersearch.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ss_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("ss_create", "ss_destroy", "ss_load_graph", "ss_set_graph", "ss_search",
+                     "ss_search_device", "ss_bruteforce", "ss_last_error", "ss_fbin_read",
+                     "ss_gt_read", "ss_nsg_read", "ss_build_graph"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    from scattersearch_amd import _lib
+    missing = [n for n in declared_functions() if not hasattr(_lib.lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_dim_padding():
+    from scattersearch_amd import dim_padded
+    assert [dim_padded(d) for d in (1, 8, 96, 100, 128, 200)] == [8, 8, 96, 104, 128, 200]
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_create_without_gpu_reports_error():
+    from scattersearch_amd import Index, SSError
+    from scattersearch_amd._lib import SS_EDEVICE
+    with pytest.raises(SSError) as e:
+        Index(np.zeros((4, 8), dtype=np.float32))
+    assert e.value.code == SS_EDEVICE
+    assert "device" in str(e.value)
+
+
+def test_invalid_create_arguments():
+    from scattersearch_amd._lib import lib, SS_EINVAL
+    h = ctypes.c_void_p()
+    base = np.zeros((4, 8), dtype=np.float32)
+    p = base.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    assert lib.ss_create(8, 4, p, 7, ctypes.byref(h)) == SS_EINVAL  # unknown metric
+    assert lib.ss_create(0, 4, p, 0, ctypes.byref(h)) == SS_EINVAL
+    assert lib.ss_create(8, 0, p, 0, ctypes.byref(h)) == SS_EINVAL
+    assert lib.ss_create(8, 4, None, 0, ctypes.byref(h)) == SS_EINVAL
+    assert b"ss_create" in lib.ss_last_error()
+
+
+def test_null_index_calls_fail_cleanly():
+    from scattersearch_amd._lib import lib, SS_EINVAL
+    assert lib.ss_destroy(None) == 0
+    assert lib.ss_set_graph(None, None, 4, 0) == SS_EINVAL
+    assert lib.ss_search(None, None, 1, None, None, 1, 1, 1, None, None, None, None) == SS_EINVAL

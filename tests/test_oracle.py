@@ -1,0 +1,233 @@
+"""Pins the CPU oracle: against the reference's own compiled routines where
+they are built (oracle/_ref), against closed forms, brute force and the
+expand-then-merge model on tiny instances."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_graph
+from merge_model import merge_search
+
+NONE = 0xFFFFFFFF
+DIMS = [8, 16, 24, 96, 128, 200, 256]
+
+
+def _ref_or_skip():
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref/libref.so not built (needs the reference sources)")
+    return R
+
+
+@pytest.mark.parametrize("dim", DIMS)
+def test_distances_bit_exact_with_reference_build(dim):
+    R = _ref_or_skip()
+    rng = np.random.default_rng(dim)
+    fp = ctypes.POINTER(ctypes.c_float)
+    for _ in range(400):
+        a = (rng.standard_normal(dim + 8) * rng.uniform(0.01, 100)).astype(np.float32)
+        b = (rng.standard_normal(dim + 8) * rng.uniform(0.01, 100)).astype(np.float32)
+        pa, pb = a.ctypes.data_as(fp), b.ctypes.data_as(fp)
+        assert np.float32(R.ref_distance_l2sqr_avx(pa, pb, dim)) == np.float32(oracle.dist(0, a[:dim], b[:dim]))
+        assert np.float32(R.ref_distance_ip_avx(pa, pb, dim)) == np.float32(oracle.dist(1, a[:dim], b[:dim]))
+
+
+@pytest.mark.parametrize("dim", DIMS + [20, 100])
+def test_distances_closed_form(dim):
+    rng = np.random.default_rng(100 + dim)
+    for _ in range(200):
+        a = rng.standard_normal(dim).astype(np.float32)
+        b = rng.standard_normal(dim).astype(np.float32)
+        l2 = float(np.sum((a.astype(np.float64) - b) ** 2))
+        ip = -float(np.dot(a.astype(np.float64), b))
+        assert abs(oracle.dist(0, a, b) - l2) <= 1e-5 * l2
+        assert abs(oracle.dist(1, a, b) - ip) <= 1e-5 * np.sum(np.abs(a.astype(np.float64) * b))
+
+
+def test_distance_special_cases():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(128).astype(np.float32)
+    assert oracle.dist(0, a, a) == 0.0
+    e0 = np.zeros(128, np.float32)
+    e1 = np.zeros(128, np.float32)
+    e0[3], e1[77] = 2.0, 5.0
+    assert oracle.dist(1, e0, e1) == 0.0
+    assert oracle.dist(0, e0, e1) == 29.0
+    assert oracle.dist(1, a, a) == pytest.approx(-float(np.dot(a.astype(np.float64), a)), rel=1e-6)
+    # the tail block (dim = 8 mod 16) is included
+    b = np.zeros(200, np.float32)
+    b[199] = 3.0
+    assert oracle.dist(0, b, np.zeros(200, np.float32)) == 9.0
+
+
+def _py_insert_sequence(pool_ids, pool_d, t, L, ids, ds, use_filter):
+    pool = (oracle.Neighbor * (L + 1))()
+    for i in range(L + 1):
+        pool[i] = oracle.Neighbor(int(pool_ids[i]), float(pool_d[i]), 1)
+    out = []
+    for i, d in zip(ids, ds):
+        if use_filter and d >= pool[t - 1].distance:
+            out.append(-1)
+            continue
+        out.append(oracle.insert_into_pool(pool, t, oracle.Neighbor(int(i), float(d), 1)))
+        if t < L:
+            t += 1
+    return [p.id for p in pool], [p.distance for p in pool], t, out
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_insert_into_pool_matches_reference(seed):
+    R = _ref_or_skip()
+    rng = np.random.default_rng(seed)
+    L = int(rng.integers(1, 40))
+    t = int(rng.integers(1, L + 1))
+    grid = rng.integers(1, 4)  # coarse values make equal distances common
+    pd = np.sort(np.round(rng.uniform(0, 10, L + 1) * grid) / grid).astype(np.float32)
+    pi = rng.permutation(1000)[:L + 1].astype(np.uint32)
+    n = int(rng.integers(1, 60))
+    ds = (np.round(rng.uniform(-1, 11, n) * grid) / grid).astype(np.float32)
+    ids = rng.integers(0, 1000, n).astype(np.uint32)
+    dup = rng.random(n) < 0.2
+    if dup.any():  # some repeat a pooled (id, distance) pair: InsertIntoPool returns K+1
+        j = rng.integers(0, t, dup.sum())
+        ids[dup], ds[dup] = pi[j], pd[j]
+    use_filter = int(seed % 2)
+    p_ids, p_d, t1, pos = _py_insert_sequence(pi, pd, t, L, ids, ds, use_filter)
+    r_ids, r_d, r_u = pi.copy(), pd.copy(), np.ones(L + 1, np.uint8)
+    rt = ctypes.c_uint32(t)
+    r_pos = np.empty(n, np.int32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    R.ref_insert_sequence(r_ids.ctypes.data_as(u32p), r_d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                          r_u.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(rt), L,
+                          ids.ctypes.data_as(u32p), ds.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n,
+                          r_pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), use_filter)
+    assert pos == r_pos.tolist()
+    assert t1 == rt.value
+    assert p_ids == r_ids.tolist()
+    assert np.array_equal(np.array(p_d, np.float32), r_d)
+
+
+def test_insert_into_pool_brute_force_model():
+    """Without the reference build: every accepted insert lands after all
+    entries with a distance <= its own (closed form of the binary search)."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        L = int(rng.integers(1, 20))
+        pd = np.sort(rng.integers(0, 6, L + 1).astype(np.float32))
+        pi = rng.permutation(100)[:L + 1]
+        d = np.float32(rng.integers(0, 6) + 0.5 * rng.integers(0, 2))
+        pool = (oracle.Neighbor * (L + 1))()
+        for i in range(L + 1):
+            pool[i] = oracle.Neighbor(int(pi[i]), float(pd[i]), 1)
+        new_id = 1000
+        r = oracle.insert_into_pool(pool, L, oracle.Neighbor(new_id, float(d), 1))
+        if pd[0] > d:
+            assert r == 0
+        elif pd[L - 1] < d:
+            assert r == L
+        else:
+            # after every entry <= d; an entry equal to the last one goes in
+            # front of it (the search's strict tail filter never asks this)
+            assert r == min(int(np.sum(pd[:L] <= d)), L - 1)
+        assert pool[r].id == new_id
+
+
+def _dist_fn(metric):
+    return lambda a, b: np.float32(oracle.dist(metric, a, b))
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_merge_model_equals_sequential_search(seed):
+    """The device's expand-then-merge (stable merge of the pool and the row's
+    admitted candidates, P first on ties, candidates in row order) gives the
+    sequential reference loop's ids, distances, distance count and hops."""
+    rng = np.random.default_rng(seed)
+    num, dim = int(rng.integers(20, 90)), 8
+    if seed % 3 == 0:
+        base = rng.integers(-2, 3, (num, dim)).astype(np.float32)  # many exact ties
+    else:
+        base = rng.standard_normal((num, dim)).astype(np.float32)
+    adj = random_graph(rng, num, int(rng.integers(2, 12)))
+    ep = int(rng.integers(0, num))
+    metric = seed % 2
+    for _ in range(4):
+        q = rng.integers(-2, 3, dim).astype(np.float32) if seed % 3 == 0 else \
+            rng.standard_normal(dim).astype(np.float32)
+        L = int(rng.integers(1, 24))
+        K = int(rng.integers(1, L + 1))
+        o = oracle.search(base, adj, ep, q, K, L, metric)
+        m = merge_search(base, adj, ep, q, K, L, _dist_fn(metric))
+        assert np.array_equal(o[0], m[0]), (seed, L, K)
+        assert np.array_equal(o[1].view(np.uint32), m[1].view(np.uint32))
+        assert o[2] == m[2] and o[3] == m[3]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_forgetful_visited_set_changes_only_distance_count(seed):
+    rng = np.random.default_rng(1000 + seed)
+    num, dim = 80, 8
+    base = rng.standard_normal((num, dim)).astype(np.float32)
+    adj = random_graph(rng, num, 10)
+    ep = 0
+    frng = np.random.default_rng(seed)
+    for _ in range(5):
+        q = rng.standard_normal(dim).astype(np.float32)
+        L = int(rng.integers(2, 30))
+        o = oracle.search(base, adj, ep, q, 5 if L >= 5 else L, L)
+        m = merge_search(base, adj, ep, q, 5 if L >= 5 else L, L, _dist_fn(0),
+                         forget=lambda x: frng.random() < 0.5)
+        assert np.array_equal(o[0], m[0])
+        assert np.array_equal(o[1].view(np.uint32), m[1].view(np.uint32))
+        assert o[3] == m[3]
+        assert m[2] >= o[2]
+
+
+def test_complete_graph_search_is_exact():
+    """On a complete graph with L >= N the pool ends up holding every point,
+    so the search returns the exact top-K."""
+    rng = np.random.default_rng(3)
+    num = 50
+    base = rng.standard_normal((num, 12)).astype(np.float32)
+    adj = np.full((num, 64), NONE, dtype=np.uint32)
+    for n in range(num):
+        others = [m for m in range(num) if m != n]
+        adj[n, :len(others)] = rng.permutation(others)
+    q = rng.standard_normal((20, 12)).astype(np.float32)
+    ids, ds, ndc, hops = oracle.search_batch(base, adj, 7, q, 10, 64)
+    bf, bd = oracle.bruteforce(base, q, 10)
+    assert np.array_equal(ids, bf)
+    assert np.array_equal(ds.view(np.uint32), bd.view(np.uint32))
+    assert (ndc == num).all()  # ep itself is reached through a neighbour
+
+
+def test_search_invariants(c0):
+    base, adj, ep = c0["base"], c0["adj"], c0["ep"]
+    ids, ds, ndc, hops = oracle.search_batch(base, adj, ep, c0["queries"], 32, 64)
+    for r in range(ids.shape[0]):
+        assert (np.diff(ds[r]) >= 0).all()
+        assert len(set(ids[r].tolist())) == ids.shape[1]
+        for j in range(0, 32, 7):
+            assert ds[r, j] == np.float32(oracle.dist(0, c0["queries"][r], base[ids[r, j]]))
+    assert (hops <= ndc).all() and (hops > 0).all()
+
+
+def test_bruteforce_closed_form():
+    rng = np.random.default_rng(2)
+    base = rng.standard_normal((700, 24)).astype(np.float32)
+    q = rng.standard_normal((9, 24)).astype(np.float32)
+    ids, ds = oracle.bruteforce(base, q, 15)
+    for r in range(9):
+        d = np.array([oracle.dist(0, q[r], b) for b in base], dtype=np.float32)
+        order = np.lexsort((np.arange(700), d))[:15]
+        assert np.array_equal(ids[r], order)
+        assert np.array_equal(ds[r], d[order])
+
+
+def test_scatter_search_runs_and_finds_neighbours(c0):
+    gt, _ = oracle.bruteforce(c0["base"], c0["queries"][:40], 10)
+    ids, lat, ndc = oracle.scatter_search(c0["base"], c0["adj"], c0["ep"], c0["queries"][:40], 10, 64, 4)
+    assert ids.shape == (40, 10) and ndc > 0 and (lat >= 0).all()
+    from scattersearch_amd import metrics
+    assert metrics.recall(ids, gt, 10) > 0.3

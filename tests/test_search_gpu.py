@@ -1,0 +1,207 @@
+"""GPU parity of the search kernel against the CPU oracle
+(SearchArraySimulation, src/annsearch.cpp:182-259).
+
+The kernel computes distances with the reference's own arithmetic and
+reproduces the sequential pool semantics exactly, so on the same graph,
+entry point, L and K the ids, the distances (bit for bit), the distance
+counts and the hop counts must all equal the oracle's.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+
+NONE = 0xFFFFFFFF
+
+
+def _index(base, adj, ep, metric=0):
+    from scattersearch_amd import Index
+    ix = Index(base, metric)
+    ix.set_graph(adj, ep)
+    return ix
+
+
+def _assert_same(gpu, ora, check_ndc=True, what=""):
+    ids, dists, st = gpu
+    o_ids, o_d, o_ndc, o_hops = ora
+    k = o_ids.shape[1]
+    bad = np.nonzero((ids[:, :k] != o_ids).any(1))[0]
+    assert bad.size == 0, f"{what}: ids differ on queries {bad[:10]}"
+    db = dists[:, :k].view(np.uint32) != o_d.view(np.uint32)
+    assert not db.any(), f"{what}: distances differ on queries {np.nonzero(db.any(1))[0][:10]}"
+    assert np.array_equal(st["hops"], o_hops), f"{what}: hop counts differ"
+    if check_ndc:
+        assert np.array_equal(st["ndc"], o_ndc), f"{what}: distance counts differ"
+
+
+def test_c0_parity_bit_exact(c0):
+    import oracle
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    gpu = ix.search(c0["queries"], 10, 64, visited_slots=16384)
+    assert gpu[2]["lossy"].sum() == 0
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], c0["queries"], 10, 64)
+    _assert_same(gpu, ora, what="c0 L=64")
+
+
+@pytest.mark.parametrize("L", [10, 32, 128, 256, 512])
+def test_c0_pool_sizes(c0, L):
+    import oracle
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    K = min(10, L)
+    gpu = ix.search(c0["queries"][:100], K, L, visited_slots=32768 if L >= 256 else 16384)
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], c0["queries"][:100], K, L)
+    _assert_same(gpu, ora, check_ndc=gpu[2]["lossy"].sum() == 0, what=f"L={L}")
+
+
+def test_per_query_launch_equals_batched(small):
+    ix = _index(small["base"], small["adj"], small["ep"])
+    from scattersearch_amd import SS_MODE_PER_QUERY
+    a = ix.search(small["queries"], 10, 48)
+    b = ix.search(small["queries"], 10, 48, mode=SS_MODE_PER_QUERY)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("wpb", [1, 2, 4, 8])
+def test_launch_shapes_same_results(small, wpb):
+    ix = _index(small["base"], small["adj"], small["ep"])
+    ref = ix.search(small["queries"], 10, 48, waves_per_block=1, blocks=1)
+    out = ix.search(small["queries"], 10, 48, waves_per_block=wpb)
+    assert np.array_equal(ref[0], out[0])
+    assert np.array_equal(ref[2], out[2])
+
+
+def test_uneven_k_and_l_in_one_launch(small):
+    import oracle
+    rng = np.random.default_rng(3)
+    nq = small["queries"].shape[0]
+    K = rng.choice([1, 10, 50], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(k, 257) for k in K], dtype=np.uint32)
+    ix = _index(small["base"], small["adj"], small["ep"])
+    ids, d, st = ix.search(small["queries"], K, L, k_stride=64, visited_slots=16384)
+    for q in range(nq):
+        o_ids, o_d, o_ndc, o_hops = oracle.search(small["base"], small["adj"], small["ep"],
+                                                  small["queries"][q], int(K[q]), int(L[q]))
+        k = int(K[q])
+        assert np.array_equal(ids[q, :k], o_ids), f"query {q} K={k} L={L[q]}"
+        assert np.array_equal(d[q, :k].view(np.uint32), o_d.view(np.uint32))
+        assert (ids[q, k:] == NONE).all() and np.isinf(d[q, k:]).all()
+        assert st["hops"][q] == o_hops
+        if st["lossy"][q] == 0:
+            assert st["ndc"][q] == o_ndc
+
+
+@pytest.mark.parametrize("slots", [256, 512, 1024])
+def test_lossy_visited_hash_keeps_results(c0, slots):
+    """A visited hash too small to remember everything only costs repeated
+    distance computations: ids, distances and hops are unchanged."""
+    import oracle
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    gpu = ix.search(c0["queries"], 10, 128, visited_slots=slots)
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], c0["queries"], 10, 128)
+    _assert_same(gpu, ora, check_ndc=False, what=f"slots={slots}")
+    assert (gpu[2]["ndc"] >= ora[2]).all()
+    if slots == 256:
+        assert gpu[2]["lossy"].sum() > 0
+
+
+@pytest.mark.parametrize("dim", [200, 96, 40, 20, 8])
+@pytest.mark.parametrize("metric", [0, 1])
+def test_dims_and_metrics(dim, metric):
+    import oracle
+    from scattersearch_amd import data
+    base, q = data.mixture(2000, 40, dim, 6, 0.3, seed=dim + 100 * metric,
+                           lognormal_sigma=0.5 if metric else 0.0, unit_norm=bool(metric))
+    adj, ep, _ = oracle.cpu_graph(base, metric, 20, 20, 24, passes=1)
+    ix = _index(base, adj, ep, metric)
+    gpu = ix.search(q, 10, 40, visited_slots=8192)
+    ora = oracle.search_batch(base, adj, ep, q, 10, 40, metric=metric)
+    _assert_same(gpu, ora, what=f"dim={dim} metric={metric}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_graphs_with_ties(seed):
+    """Random (non-navigable) graphs over data with many repeated vectors, so
+    that exact distance ties reach the pool order, the tail filter and the
+    merge."""
+    import oracle
+    rng = np.random.default_rng(seed)
+    num = 600
+    distinct = rng.standard_normal((60, 16)).astype(np.float32)
+    base = distinct[rng.integers(0, 60, size=num)]
+    if seed % 2:
+        base = np.round(base * 2) / 2  # coarse grid: more ties
+    q = rng.standard_normal((30, 16)).astype(np.float32)
+    r_pad = int(rng.choice([8, 32, 64]))
+    adj = random_graph(rng, num, r_pad)
+    ep = int(rng.integers(0, num))
+    ix = _index(base, adj, ep)
+    for L in (4, 16, 70):
+        K = min(L, 8)
+        gpu = ix.search(q, K, L, visited_slots=4096)
+        ora = oracle.search_batch(base, adj, ep, q, K, L)
+        _assert_same(gpu, ora, what=f"seed={seed} L={L}")
+
+
+def test_small_entry_degree_fills_with_none():
+    """deg(ep) < K: only improving candidates grow the pool, unfilled slots
+    come back as (SS_NONE, +inf)."""
+    import oracle
+    rng = np.random.default_rng(11)
+    base = rng.standard_normal((50, 16)).astype(np.float32)
+    adj = np.full((50, 4), NONE, dtype=np.uint32)
+    adj[0, :2] = [1, 2]  # entry point with two neighbours, nodes 1, 2 lead nowhere
+    ix = _index(base, adj, 0)
+    ids, d, st = ix.search(base[5:8], 10, 20)
+    o = oracle.search_batch(base, adj, 0, base[5:8], 10, 20)
+    assert np.array_equal(ids, o[0])
+    assert (ids[:, 2:] == NONE).all() and np.isinf(d[:, 2:]).all()
+
+
+def test_hipgraph_replay_matches_eager(small):
+    ix = _index(small["base"], small["adj"], small["ep"])
+    eager = ix.search(small["queries"], 10, 48)
+    for _ in range(4):
+        g = ix.search(small["queries"], 10, 48, use_graph=True)
+        assert np.array_equal(eager[0], g[0])
+        assert np.array_equal(eager[2], g[2])
+        mid = ix.search(small["queries"][:7], 5, 20)  # eager work between replays
+        assert mid[0].shape == (7, 5)
+
+
+def test_device_buffers_api(small):
+    torch = pytest.importorskip("torch")
+    from scattersearch_amd import dim_padded, STATS_DTYPE
+    ix = _index(small["base"], small["adj"], small["ep"])
+    q = small["queries"]
+    nq, dim = q.shape
+    dq = torch.zeros((nq, dim_padded(dim)), dtype=torch.float32, device="cuda")
+    dq[:, :dim] = torch.from_numpy(q).cuda()
+    ids = torch.empty((nq, 16), dtype=torch.int32, device="cuda")
+    dd = torch.empty((nq, 16), dtype=torch.float32, device="cuda")
+    st = torch.empty((nq, 4), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+    ix.search_device(dq.data_ptr(), nq, 10, 48, 16, ids.data_ptr(), dd.data_ptr(), st.data_ptr(),
+                     stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    host = ix.search(q, 10, 48, k_stride=16)
+    assert np.array_equal(ids.cpu().numpy().view(np.uint32), host[0])
+    assert np.array_equal(st.cpu().numpy().view(np.uint32).reshape(-1).view(STATS_DTYPE), host[2])
+
+
+def test_invalid_arguments_rejected(small):
+    from scattersearch_amd import SSError
+    ix = _index(small["base"], small["adj"], small["ep"])
+    with pytest.raises(SSError):
+        ix.search(small["queries"], 20, 10)  # K > L
+    with pytest.raises(SSError):
+        ix.search(small["queries"], 10, 1024)  # L above the pool bound
+    with pytest.raises(SSError):
+        ix.search(small["queries"], 10, 48, visited_slots=1000)  # not a power of two
+    bad = small["adj"].copy()
+    bad[3, 0] = small["base"].shape[0] + 5
+    with pytest.raises(SSError):
+        ix.set_graph(bad, small["ep"])
