@@ -1,0 +1,275 @@
+#!/usr/bin/env python3
+"""Benchmark of the graph-ANN query path on one MI355X.
+
+Workload (default): the 1M x 128 fp32 Gaussian mixture (1,000 components,
+sigma 0.25, seeded), NSG-style graph with R=32 built on the GPU, 10,000
+queries, L2, K=10.  The L sweep {32, 64, 128, 256} picks the smallest L
+whose recall@10 against exact ground truth reaches 0.95; at that L:
+
+  value        whole-batch QPS of the batched persistent kernel (one
+               hipGraph-replayed launch over all queries per step, inputs
+               resident in HBM), timed over --steps steps;
+  plain_qps    the same queries through one kernel launch per query;
+  latency      p50/p99 of per-query device service time in the batch;
+  roofline     algorithmic bytes per launch (sum over queries of
+               NDC*dim*4 + hops*R*4 + dim*4 + K*8) / kernel time vs 8 TB/s;
+  cpu_baseline the ScatterSearch intra-query search (CPU restatement) on
+               the host cores over a bounded sample of the same queries.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC_NAME = "QPS at recall@10=0.95, batched vs plain kernel, 1 MI355X; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--num", type=int, default=0, help="override the base size")
+    ap.add_argument("--nq", type=int, default=0, help="override the query count")
+    ap.add_argument("--L", type=int, default=0, help="fixed L instead of the sweep")
+    ap.add_argument("--recall-target", type=float, default=0.95)
+    ap.add_argument("--plain-queries", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--visited-slots", type=int, default=0)
+    ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--profile", action="store_true",
+                    help="only build, then run --steps batched launches (for rocprofv3)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the query path "
+              "shards by query across GPUs with no exchange, so N GPUs run N independent copies)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # noqa: F401  (shares its HIP runtime with the library)
+    import scattersearch_amd as ss
+    from scattersearch_amd import data, metrics
+    from scattersearch_amd.graph import gpu_build_graph
+
+    torch.cuda.set_device(0)
+    w = data.WORKLOADS[args.workload]
+    num = args.num or w.num
+    nq = args.nq or w.nq
+    t0 = time.time()
+    base, queries = data.generate(w, num, nq)
+    log(f"data {num}x{w.dim} + {nq} queries in {time.time() - t0:.1f}s")
+
+    ix = ss.Index(base, w.metric)
+    t0 = time.time()
+    adj, ep, ginfo = gpu_build_graph(ix, base, w.R, w.knn_k, w.nav_L, log=log)
+    log(f"graph built in {time.time() - t0:.1f}s, ep {ep}")
+    r_pad = ix.info()["r_pad"]
+    K = w.K
+
+    dim_pad = ss.dim_padded(w.dim)
+    dq = torch.zeros((nq, dim_pad), dtype=torch.float32, device="cuda")
+    dq[:, :w.dim] = torch.from_numpy(queries).cuda()
+    d_ids = torch.empty((nq, K), dtype=torch.int32, device="cuda")
+    d_dist = torch.empty((nq, K), dtype=torch.float32, device="cuda")
+    d_stats = torch.empty((nq, 8), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()  # default priority; hipGraph capture needs a non-null stream
+
+    def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block):
+        ix.search_device(dq.data_ptr(), nq, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
+                         d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=use_graph,
+                         visited_slots=slots, waves_per_block=wpb)
+
+    def stats_host():
+        return d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
+
+    if args.profile:
+        L = args.L or 64
+        for _ in range(args.steps):
+            launch(L)
+        torch.cuda.synchronize()
+        log(f"profile: {args.steps} launches at L={L} done")
+        return
+
+    t0 = time.time()
+    gt, _ = ix.bruteforce(queries, K)
+    log(f"ground truth in {time.time() - t0:.1f}s")
+
+    def timed(L, steps, warmup):
+        for _ in range(warmup):
+            launch(L)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t_wall = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            launch(L)
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_wall
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        return wall, kern_ms
+
+    sweep = []
+    L_list = [args.L] if args.L else list(w.L_list)
+    chosen = None
+    for L in L_list:
+        wall, kern_ms = timed(L, 3, 1)
+        ids = d_ids.cpu().numpy().view(np.uint32)
+        st = stats_host()
+        rec = metrics.recall(ids, gt, K)
+        row = dict(L=L, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
+                   mean_ndc=round(float(st["ndc"].mean()), 1), mean_hops=round(float(st["hops"].mean()), 1),
+                   lossy=int(st["lossy"].sum()))
+        sweep.append(row)
+        log(f"sweep {row}")
+        if chosen is None and rec >= args.recall_target:
+            chosen = L
+    if chosen is None:
+        chosen = L_list[-1]
+        log(f"no L reached recall {args.recall_target}; reporting the largest L")
+    L = chosen
+
+    # Timed region: K steps of one batched launch over all queries.
+    wall, kern_ms = timed(L, args.steps, args.warmup)
+    qps = nq * args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+    ids = d_ids.cpu().numpy().view(np.uint32)
+    st = stats_host()
+    rec = metrics.recall(ids, gt, K)
+    khz = ix.wall_clock_khz()
+    service_ms = (st["t_end"].astype(np.float64) - st["t_start"]) / khz
+    done_ms = (st["t_end"].astype(np.float64) - st["t_start"].min()) / khz
+    bq = metrics.bytes_per_query(st["ndc"], st["hops"], w.dim, r_pad, K)
+    launch_bytes = float(bq.sum())
+    achieved_gbs = launch_bytes / (kern_ms * 1e-3) / 1e9
+    log(f"batched L={L}: {qps:.0f} QPS, kernel {kern_ms:.3f} ms, {achieved_gbs:.0f} GB/s algorithmic, "
+        f"recall {rec:.4f}")
+
+    # Plain kernel: one launch per query.
+    npq = min(args.plain_queries, nq)
+    from scattersearch_amd import SS_MODE_PER_QUERY
+
+    def plain(n):
+        ix.search_device(dq.data_ptr(), n, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
+                         d_stats.data_ptr(), stream=stream.cuda_stream, mode=SS_MODE_PER_QUERY,
+                         visited_slots=args.visited_slots)
+
+    plain(min(64, npq))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plain(npq)
+    torch.cuda.synchronize()
+    plain_qps = npq / (time.perf_counter() - t)
+    log(f"plain per-query launches: {plain_qps:.0f} QPS over {npq} queries")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(base, adj, ep, queries, gt, K, L, w.metric, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC_NAME,
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: seeded {w.centres}-component Gaussian mixture (seed {w.seed}), graph built on the GPU",
+        "config": {"workload": w.name, "num": num, "dim": w.dim, "nq": nq, "R": w.R, "K": K, "L": L,
+                   "metric": "L2" if w.metric == 0 else "inner_product", "recall@10": round(rec, 4),
+                   "mean_ndc": round(float(st["ndc"].mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1)},
+        "plain_qps": round(plain_qps, 1),
+        "speedup_vs_plain": round(qps / plain_qps, 2),
+        "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
+                       "service_p99": round(metrics.percentile(service_ms, 0.99), 4),
+                       "batch_completion_p50": round(metrics.percentile(done_ms, 0.5), 4),
+                       "batch_completion_p99": round(metrics.percentile(done_ms, 0.99), 4)},
+        "l_sweep": sweep,
+        "graph": {"mean_degree": ginfo["passes"][-1]["mean_degree"], "knn_s": round(ginfo["knn_s"], 1),
+                  "passes": len(ginfo["passes"])},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": round(launch_bytes), "kernel_ms": round(kern_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):
+    """ScatterSearch (MultiThreadSearchArraySimulationWithET) restated on the
+    host: queries one after another, T = host threads per query; plus the
+    serial search run inter-query parallel.  Bounded samples of the same
+    queries, each about `seconds` of CPU wall time."""
+    import oracle
+    from scattersearch_amd import metrics
+    T = host_threads()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    # calibrate the sample size on 20 queries
+    n0 = min(20, queries.shape[0])
+    t = time.perf_counter()
+    oracle.scatter_search(base, adj, ep, queries[:n0], K, L, T, metric)
+    per_q = (time.perf_counter() - t) / n0
+    n = int(min(queries.shape[0], max(n0, seconds / max(per_q, 1e-6))))
+    t = time.perf_counter()
+    ids, lat, ndc = oracle.scatter_search(base, adj, ep, queries[:n], K, L, T, metric)
+    wall = time.perf_counter() - t
+    et_qps = n / wall
+    et_rec = metrics.recall(ids, gt[:n], K)
+    log(f"cpu ScatterSearch T={T}: {et_qps:.0f} QPS over {n} queries, recall {et_rec:.4f}")
+    # serial search, inter-query parallel
+    t = time.perf_counter()
+    oracle.search_batch(base, adj, ep, queries[:n0], K, L, metric, nthreads=T)
+    per_q = (time.perf_counter() - t) / n0
+    n2 = int(min(queries.shape[0], max(n0, seconds / max(per_q, 1e-6))))
+    t = time.perf_counter()
+    sids = oracle.search_batch(base, adj, ep, queries[:n2], K, L, metric, nthreads=T)[0]
+    ser_qps = n2 / (time.perf_counter() - t)
+    log(f"cpu serial search x{T} threads: {ser_qps:.0f} QPS over {n2} queries")
+    return {"value": round(et_qps, 1), "unit": "queries/s", "cores": T, "kind": "port",
+            "sample": f"first {n} of the same queries at L={L}, K={K}, one after another, "
+                      f"{T} intra-query threads (ScatterSearch with election at hop 50)",
+            "recall@10": round(et_rec, 4), "cpu": model,
+            "serial_search_qps": round(ser_qps, 1),
+            "serial_search_sample": f"first {n2} queries, {T} queries in parallel, private visited sets",
+            "serial_search_recall@10": round(metrics.recall(sids, gt[:n2], K), 4)}
+
+
+if __name__ == "__main__":
+    main()

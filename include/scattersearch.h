@@ -61,10 +61,12 @@ typedef struct ss_search_params {
 
 /* Per-query counters written by every search. */
 typedef struct ss_query_stats {
-    uint32_t ndc;   /* distance computations, including the seed ones */
-    uint32_t hops;  /* expanded nodes */
-    uint32_t lossy; /* visited-hash inserts that found no free slot */
+    uint32_t ndc;     /* distance computations, including the seed ones */
+    uint32_t hops;    /* expanded nodes */
+    uint32_t lossy;   /* visited-hash inserts that found no free slot */
     uint32_t reserved;
+    uint64_t t_start; /* device wall clock when a wavefront took the query */
+    uint64_t t_end;   /* device wall clock when its results were written */
 } ss_query_stats;
 
 const char *ss_last_error(void);
@@ -82,6 +84,9 @@ int ss_create_from_device(uint32_t dim, uint32_t num, const float *d_base, int m
 int ss_destroy(ss_index *index);
 
 uint32_t ss_dim_padded(uint32_t dim);
+
+/* Frequency of the device wall clock behind ss_query_stats.t_start/t_end. */
+int ss_wall_clock_khz(const ss_index *index, uint32_t *khz);
 int ss_index_info(const ss_index *index, uint32_t *dim, uint32_t *num, uint32_t *r_pad,
                   uint32_t *ep, int *metric);
 
@@ -111,6 +116,16 @@ int ss_search(ss_index *index, const float *queries, uint32_t nq,
               const uint32_t *K, const uint32_t *L, uint32_t k_default, uint32_t l_default,
               uint32_t k_stride, const ss_search_params *params,
               uint32_t *out_ids, float *out_dists, ss_query_stats *stats);
+
+/* ss_search with one K and L for all queries that also returns, per query,
+ * the ids of the expanded nodes in expansion order ([nq][path_cap], SS_NONE
+ * after the last): the visit list of the reference's COLLECT_VISITED_ID
+ * instrumentation (include/annsearch.h:18, src/annsearch.cpp:581-583).
+ * Graph construction takes its navigation candidates from it. */
+int ss_search_path(ss_index *index, const float *queries, uint32_t nq, uint32_t K, uint32_t L,
+                   uint32_t k_stride, const ss_search_params *params, uint32_t path_cap,
+                   uint32_t *out_ids, float *out_dists, ss_query_stats *stats,
+                   uint32_t *out_path);
 
 /* Same with every buffer already on the device: d_queries is
  * [nq][dim_pad]; k_max / l_max bound K[q] / L[q] (they size on-chip state);

@@ -48,11 +48,11 @@ def lib():
         L.ora_insert_into_pool.argtypes = [ctypes.POINTER(Neighbor), _u32, Neighbor]
         L.ora_search.restype = ctypes.c_int
         L.ora_search.argtypes = [ctypes.POINTER(Graph), _f32p, _u32, ctypes.c_int, _f32p, _u32, _u32,
-                                 _u32p, _f32p, _u32p, _u32p, _u8p]
+                                 _u32p, _f32p, _u32p, _u32p, _u8p, _u32p, _u32]
         L.ora_search_batch.restype = ctypes.c_int
         L.ora_search_batch.argtypes = [ctypes.POINTER(Graph), _f32p, _u32, ctypes.c_int, _f32p, _u32,
                                        _u32p, _u32p, _u32, _u32p, _f32p, _u32p, _u32p, ctypes.c_int,
-                                       _f32p]
+                                       _f32p, _u32p, _u32]
         L.ora_bruteforce.restype = ctypes.c_int
         L.ora_bruteforce.argtypes = [_f32p, _u32, _f32p, _u32, _u32, ctypes.c_int, _u32, _u32p, _u32p,
                                      _f32p, ctypes.c_int]
@@ -95,16 +95,17 @@ def search(base, adj, ep, query, K, L, metric=L2):
     ndc, hops = _u32(), _u32()
     vis = np.zeros(base.shape[0], dtype=np.uint8)
     rc = lib().ora_search(ctypes.byref(g), _f(base), base.shape[1], metric, _f(q), K, L, _u(ids), _f(ds),
-                          ctypes.byref(ndc), ctypes.byref(hops), vis.ctypes.data_as(_u8p))
+                          ctypes.byref(ndc), ctypes.byref(hops), vis.ctypes.data_as(_u8p), None, 0)
     if rc:
         raise RuntimeError(f"ora_search failed ({rc})")
     return ids, ds, ndc.value, hops.value
 
 
 def search_batch(base, adj, ep, queries, K, L, metric=L2, nthreads=0, k_stride=None,
-                 latency=False):
+                 latency=False, path_cap=0):
     """Many queries (K, L int or per-query); returns ids, dists, ndc, hops
-    (and per-query latency in ms when asked)."""
+    (then per-query latency in ms when asked, then the expanded-node lists
+    [nq][path_cap] when path_cap > 0)."""
     base = np.ascontiguousarray(base, dtype=np.float32)
     Q = np.ascontiguousarray(queries, dtype=np.float32)
     nq = Q.shape[0]
@@ -119,13 +120,18 @@ def search_batch(base, adj, ep, queries, K, L, metric=L2, nthreads=0, k_stride=N
     ndc = np.zeros(nq, dtype=np.uint32)
     hops = np.zeros(nq, dtype=np.uint32)
     lat = np.zeros(nq, dtype=np.float32)
+    path = np.empty((nq, path_cap), dtype=np.uint32) if path_cap else None
     rc = lib().ora_search_batch(ctypes.byref(g), _f(base), base.shape[1], metric, _f(Q), nq, _u(Ka),
-                                _u(La), ks, _u(ids), _f(ds), _u(ndc), _u(hops), nthreads, _f(lat))
+                                _u(La), ks, _u(ids), _f(ds), _u(ndc), _u(hops), nthreads, _f(lat),
+                                _u(path), path_cap)
     if rc:
         raise RuntimeError(f"ora_search_batch failed ({rc})")
+    out = (ids, ds, ndc, hops)
     if latency:
-        return ids, ds, ndc, hops, lat
-    return ids, ds, ndc, hops
+        out = out + (lat,)
+    if path_cap:
+        out = out + (path,)
+    return out
 
 
 def bruteforce(base, queries, k, metric=L2, exclude=None, nthreads=0):
@@ -180,7 +186,7 @@ def ref_lib():
     return R
 
 
-def cpu_graph(base, metric, R, knn_k, nav_L, alpha=1.2, passes=2, nthreads=0, log=None):
+def cpu_graph(base, metric, R, knn_k, nav_L, alpha=1.0, passes=3, nthreads=0, log=None):
     """The graph construction pipeline with the oracle as k-NN and search
     provider (CPU only, for tests)."""
     from scattersearch_amd.graph import build_graph
@@ -189,7 +195,8 @@ def cpu_graph(base, metric, R, knn_k, nav_L, alpha=1.2, passes=2, nthreads=0, lo
         ex = np.arange(base.shape[0], dtype=np.uint32)
         return bruteforce(base, base, k, metric, exclude=ex, nthreads=nthreads)[0]
 
-    def search_fn(knn, ep, L):
-        return search_batch(base, knn, ep, base, L, L, metric, nthreads)[0]
+    def search_fn(adj, ep, L, path_cap):
+        r = search_batch(base, adj, ep, base, L, L, metric, nthreads, path_cap=path_cap)
+        return r[0], r[-1]
 
     return build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha, passes, nthreads, log)

@@ -168,7 +168,8 @@ static void sort_pool(ora_neighbor *p, uint32_t n)
 int ora_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
                const float *query, uint32_t K, uint32_t L,
                uint32_t *out_ids, float *out_dists,
-               uint32_t *ndc_out, uint32_t *hops_out, uint8_t *visited)
+               uint32_t *ndc_out, uint32_t *hops_out, uint8_t *visited,
+               uint32_t *path, uint32_t path_cap)
 {
     if (L == 0 || K == 0 || g->ep >= g->num)
         return -1;
@@ -197,6 +198,8 @@ int ora_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
         if (retset[k].unexplored) {                              /* :221 */
             retset[k].unexplored = 0;
             uint32_t n = retset[k].id;
+            if (path && hop < path_cap)
+                path[hop] = n;
             const uint32_t *row = g->adj + (size_t)n * g->r_pad;
             uint32_t deg = row_degree(g, n);
             for (uint32_t m = 0; m < deg; ++m) {                 /* :226 */
@@ -233,6 +236,8 @@ int ora_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
                 out_dists[i] = INFINITY;
         }
     }
+    for (uint32_t i = hop; path && i < path_cap; i++)
+        path[i] = ORA_NONE;
     memset(visited, 0, g->num);                                  /* :258 */
     if (ndc_out)
         *ndc_out = ndc;
@@ -257,7 +262,7 @@ int ora_search_batch(const ora_graph *g, const float *base, uint32_t dim, int me
                      const uint32_t *K, const uint32_t *L, uint32_t k_stride,
                      uint32_t *out_ids, float *out_dists,
                      uint32_t *ndc, uint32_t *hops, int nthreads,
-                     float *latency_ms)
+                     float *latency_ms, uint32_t *path, uint32_t path_cap)
 {
     int err = 0;
     if (nthreads <= 0)
@@ -277,7 +282,8 @@ int ora_search_batch(const ora_graph *g, const float *base, uint32_t dim, int me
             int rc = ora_search(g, base, dim, metric, queries + (size_t)q * dim, K[q], L[q],
                                 out_ids + (size_t)q * k_stride,
                                 out_dists ? out_dists + (size_t)q * k_stride : NULL,
-                                ndc ? ndc + q : NULL, hops ? hops + q : NULL, visited);
+                                ndc ? ndc + q : NULL, hops ? hops + q : NULL, visited,
+                                path ? path + (size_t)q * path_cap : NULL, path_cap);
             if (latency_ms)
                 latency_ms[q] = (float)(now_ms() - t0);
             if (rc) {

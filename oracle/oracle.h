@@ -59,11 +59,14 @@ typedef struct {
 
 /* One query of SearchArraySimulation (src/annsearch.cpp:182-259).
  * visited: caller-owned byte array of graph->num entries, all zero on entry,
- * all zero again on return (mirrors flags.reset()). */
+ * all zero again on return (mirrors flags.reset()).  path (optional): the
+ * first path_cap expanded node ids in expansion order, ORA_NONE after the
+ * last (the visit list COLLECT_VISITED_ID records, include/annsearch.h:18). */
 int ora_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
                const float *query, uint32_t K, uint32_t L,
                uint32_t *out_ids, float *out_dists,
-               uint32_t *ndc, uint32_t *hops, uint8_t *visited);
+               uint32_t *ndc, uint32_t *hops, uint8_t *visited,
+               uint32_t *path, uint32_t path_cap);
 
 /* Many queries, inter-query parallel, private visited array per thread.
  * K, L per query; outputs with row stride k_stride. */
@@ -72,7 +75,7 @@ int ora_search_batch(const ora_graph *g, const float *base, uint32_t dim, int me
                      const uint32_t *K, const uint32_t *L, uint32_t k_stride,
                      uint32_t *out_ids, float *out_dists,
                      uint32_t *ndc, uint32_t *hops, int nthreads,
-                     float *latency_ms);
+                     float *latency_ms, uint32_t *path, uint32_t path_cap);
 
 /* Exact top-k by (distance, id). exclude[q] (optional) is an id skipped for
  * query q (used to drop a point from its own neighbour list). */

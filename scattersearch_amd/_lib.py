@@ -43,6 +43,8 @@ class QueryStats(ctypes.Structure):
         ("hops", ctypes.c_uint32),
         ("lossy", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("t_start", ctypes.c_uint64),
+        ("t_end", ctypes.c_uint64),
     ]
 
 
@@ -84,6 +86,7 @@ _SIGS = {
     "ss_create_from_device": (cint, [u32, u32, vp, cint, ctypes.POINTER(vp)]),
     "ss_destroy": (cint, [vp]),
     "ss_dim_padded": (u32, [u32]),
+    "ss_wall_clock_khz": (cint, [vp, u32p]),
     "ss_index_info": (cint, [vp, u32p, u32p, u32p, u32p, ctypes.POINTER(cint)]),
     "ss_load_graph": (cint, [vp, cstr]),
     "ss_set_graph": (cint, [vp, u32p, u32, u32]),
@@ -91,6 +94,8 @@ _SIGS = {
     "ss_search": (cint, [vp, f32p, u32, u32p, u32p, u32, u32, u32,
                          ctypes.POINTER(SearchParams), u32p, f32p,
                          ctypes.POINTER(QueryStats)]),
+    "ss_search_path": (cint, [vp, f32p, u32, u32, u32, u32, ctypes.POINTER(SearchParams), u32,
+                              u32p, f32p, ctypes.POINTER(QueryStats), u32p]),
     "ss_search_device": (cint, [vp, vp, u32, vp, vp, u32, u32, u32,
                                 ctypes.POINTER(SearchParams), vp, vp, vp, vp]),
     "ss_bruteforce": (cint, [vp, f32p, u32, u32, u32p, u32p, f32p]),

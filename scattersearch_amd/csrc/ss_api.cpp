@@ -77,6 +77,7 @@ struct ss_index {
     DevBuf<uint32_t> ids;
     DevBuf<float> dists;
     DevBuf<ss_query_stats> stats;
+    DevBuf<uint32_t> path;
     // hipGraph cache of the last batched launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -237,9 +238,24 @@ int ss_destroy(ss_index *ix)
     ix->ids.release();
     ix->dists.release();
     ix->stats.release();
+    ix->path.release();
     if (ix->stream)
         (void)hipStreamDestroy(ix->stream);
     delete ix;
+    return SS_OK;
+}
+
+int ss_wall_clock_khz(const ss_index *ix, uint32_t *khz)
+{
+    if (!ix || !khz) {
+        set_error("ss_wall_clock_khz: NULL argument");
+        return SS_EINVAL;
+    }
+    int rate = 0;
+    hipError_t e = hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, ix->device);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipDeviceGetAttribute(wall clock rate)");
+    *khz = (uint32_t)rate;
     return SS_OK;
 }
 
@@ -435,7 +451,8 @@ static uint32_t log2u(uint32_t x)
 
 static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
                       uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
-                      uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st)
+                      uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st,
+                      uint32_t *d_path = nullptr, uint32_t path_cap = 0)
 {
     if (!ix->has_graph) {
         set_error("search: no graph loaded");
@@ -481,6 +498,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.out_ids = d_out_ids;
     a.out_dists = d_out_dists;
     a.stats = d_stats;
+    a.path = d_path;
+    a.path_cap = d_path ? path_cap : 0;
     hipError_t e;
     if (mode == SS_MODE_PER_QUERY) {
         a.queue = nullptr;
@@ -493,7 +512,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         return SS_OK;
     }
     a.queue = ix->d_queue;
-    const bool use_graph = p && p->use_graph;
+    const bool use_graph = p && p->use_graph && !d_path;
     if (use_graph) {
         const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
                           ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
@@ -552,9 +571,10 @@ int ss_search_device(ss_index *ix, const float *d_queries, uint32_t nq, const ui
                       reinterpret_cast<hipStream_t>(stream));
 }
 
-int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K, const uint32_t *L,
-              uint32_t k_default, uint32_t l_default, uint32_t k_stride, const ss_search_params *params,
-              uint32_t *out_ids, float *out_dists, ss_query_stats *stats)
+static int search_host(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K, const uint32_t *L,
+                       uint32_t k_default, uint32_t l_default, uint32_t k_stride, const ss_search_params *params,
+                       uint32_t *out_ids, float *out_dists, ss_query_stats *stats, uint32_t path_cap,
+                       uint32_t *out_path)
 {
     if (!ix) {
         set_error("ss_search: NULL index");
@@ -589,7 +609,8 @@ int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K
     hipError_t e;
     if ((e = ix->q.reserve((size_t)nq * ix->dim_pad)) != hipSuccess || (e = ix->kl.reserve((size_t)nq * 2)) != hipSuccess ||
         (e = ix->ids.reserve((size_t)nq * k_stride)) != hipSuccess ||
-        (e = ix->dists.reserve((size_t)nq * k_stride)) != hipSuccess || (e = ix->stats.reserve(nq)) != hipSuccess)
+        (e = ix->dists.reserve((size_t)nq * k_stride)) != hipSuccess || (e = ix->stats.reserve(nq)) != hipSuccess ||
+        (out_path && (e = ix->path.reserve((size_t)nq * path_cap)) != hipSuccess))
         return hip_fail(e, "hipMalloc(search buffers)");
     hipStream_t st = ix->stream;
     if (ix->dim_pad == ix->dim) {
@@ -614,9 +635,12 @@ int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K
             return hip_fail(e, "upload of L");
     }
     int rc = run_search(ix, ix->q.ptr, nq, dK, dL, k_max, l_max, k_stride, params, ix->ids.ptr, ix->dists.ptr,
-                        ix->stats.ptr, st);
+                        ix->stats.ptr, st, out_path ? ix->path.ptr : nullptr, path_cap);
     if (rc)
         return rc;
+    if (out_path && (e = hipMemcpyAsync(out_path, ix->path.ptr, (size_t)nq * path_cap * sizeof(uint32_t),
+                                        hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "download of expanded-node lists");
     if ((e = hipMemcpyAsync(out_ids, ix->ids.ptr, (size_t)nq * k_stride * sizeof(uint32_t), hipMemcpyDeviceToHost,
                             st)) != hipSuccess)
         return hip_fail(e, "download of ids");
@@ -630,6 +654,26 @@ int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "search");
     return SS_OK;
+}
+
+int ss_search(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K, const uint32_t *L,
+              uint32_t k_default, uint32_t l_default, uint32_t k_stride, const ss_search_params *params,
+              uint32_t *out_ids, float *out_dists, ss_query_stats *stats)
+{
+    return search_host(ix, queries, nq, K, L, k_default, l_default, k_stride, params, out_ids, out_dists, stats, 0,
+                       nullptr);
+}
+
+int ss_search_path(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uint32_t L, uint32_t k_stride,
+                   const ss_search_params *params, uint32_t path_cap, uint32_t *out_ids, float *out_dists,
+                   ss_query_stats *stats, uint32_t *out_path)
+{
+    if (!out_path || path_cap == 0) {
+        set_error("ss_search_path: no path buffer");
+        return SS_EINVAL;
+    }
+    return search_host(ix, queries, nq, nullptr, nullptr, K, L, k_stride, params, out_ids, out_dists, stats,
+                       path_cap, out_path);
 }
 
 int ss_bruteforce_device(ss_index *ix, const float *d_queries, uint32_t nq, uint32_t k, int exclude_self,

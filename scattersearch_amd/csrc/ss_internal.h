@@ -35,6 +35,8 @@ struct SearchArgs {
     float *out_dists;
     ss_query_stats *stats;
     uint32_t *queue;      // non-null: persistent workers take queries from it
+    uint32_t *path;       // optional [nq][path_cap] expanded-node ids
+    uint32_t path_cap;
 };
 
 struct BruteArgs {

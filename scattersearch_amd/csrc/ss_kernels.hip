@@ -262,6 +262,7 @@ __device__ __forceinline__ void candidate_distances(const SearchArgs &a, const W
 template <int NBLK, int TAILI, int METRIC>
 __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
 {
+    const uint64_t t_start = (uint64_t)wall_clock64();
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
@@ -278,7 +279,7 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
                 out_d[i] = __builtin_inff();
         }
         if (a.stats && lane == 0)
-            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu};
+            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, t_start, (uint64_t)wall_clock64()};
         return;
     }
 
@@ -341,8 +342,11 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         const uint32_t n = uniform(w.pool_id[k]);
         const float tail = readlane_f(w.pool_d[t - 1], 0);
         wave_sync();
-        if (lane == 0)
+        if (lane == 0) {
             w.pool_id[k] = n | kExplored;
+            if (a.path && hops < a.path_cap)
+                a.path[(size_t)q * a.path_cap + hops] = n;
+        }
         hops++;
 
         // Neighbour row (one coalesced load), visited test, compaction.
@@ -460,12 +464,15 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         if (out_d)
             out_d[i] = ok ? w.pool_d[i] : __builtin_inff();
     }
+    if (a.path)
+        for (uint32_t i = hops + lane; i < a.path_cap; i += 64)
+            a.path[(size_t)q * a.path_cap + i] = kNone;
     uint32_t lsum = lossy;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         lsum += __shfl_xor(lsum, off, 64);
     if (a.stats && lane == 0)
-        a.stats[q] = ss_query_stats{ndc, hops, lsum, 0u};
+        a.stats[q] = ss_query_stats{ndc, hops, lsum, 0u, t_start, (uint64_t)wall_clock64()};
     wave_sync();
 }
 

@@ -48,13 +48,17 @@ def reachable(adj, ep):
     return r.value
 
 
-def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.2, passes=2,
+def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.0, passes=3,
                 nthreads=0, log=None):
-    """knn_fn(k) -> [num][k] ids; search_fn(adj, ep, L) -> [num][L] pools of
-    a search for every base point over graph adj.  The first pass takes its
-    navigation candidates from a search over the k-NN graph; each further
-    pass searches the graph of the pass before (so the candidates follow the
-    long-range edges that pass created).  Returns (adj [num][R], ep, info)."""
+    """knn_fn(k) -> [num][k] ids; search_fn(adj, ep, L, path_cap) -> (pool
+    ids [num][L], expanded ids [num][path_cap]) of a search for every base
+    point over graph adj.  The first pass searches the k-NN graph; each
+    further pass searches the graph of the pass before, so its candidates
+    follow the long-range edges that pass created.  Candidates of a point =
+    its k-NN, its search pool and the nodes its search expanded (NSG's
+    search-path candidates), plus its edges from the pass before, so that
+    edges which served navigation compete again instead of being lost.
+    Returns (adj [num][R], ep, info)."""
     import time
     t0 = time.time()
     knn = knn_fn(knn_k)
@@ -62,9 +66,14 @@ def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.2, pas
     ep = medoid(base)
     info = dict(knn_s=t1 - t0, passes=[])
     adj = knn
+    path_cap = 2 * nav_L
     for p in range(max(1, passes)):
         ta = time.time()
-        nav = search_fn(adj, ep, nav_L) if nav_L else None
+        nav = None
+        if nav_L:
+            pool, path = search_fn(adj, ep, nav_L, path_cap)
+            parts = [path, pool] if p == 0 else [adj, path, pool]
+            nav = np.concatenate(parts, axis=1)
         tb = time.time()
         adj, fixed = prune_and_connect(base, metric, knn, nav, R, ep, alpha, nthreads)
         tc = time.time()
@@ -78,20 +87,22 @@ def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.2, pas
     return adj, ep, info
 
 
-def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.2, passes=2, chunk=131072, nthreads=0,
+def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.0, passes=3, chunk=131072, nthreads=0,
                     log=None):
     """build_graph with the GPU providers of an Index over `base`."""
 
     def knn_fn(k):
         return index.knn_graph(k)[0]
 
-    def search_fn(knn, ep, L):
-        index.set_graph(knn, ep)
-        out = np.empty((base.shape[0], L), dtype=np.uint32)
+    def search_fn(adj, ep, L, path_cap):
+        index.set_graph(adj, ep)
+        pool = np.empty((base.shape[0], L), dtype=np.uint32)
+        path = np.empty((base.shape[0], path_cap), dtype=np.uint32)
         for s in range(0, base.shape[0], chunk):
-            ids, _, _ = index.search(base[s:s + chunk], L, L)
-            out[s:s + chunk] = ids
-        return out
+            ids, _, _, pa = index.search_path(base[s:s + chunk], L, L, path_cap)
+            pool[s:s + chunk] = ids
+            path[s:s + chunk] = pa
+        return pool, path
 
     adj, ep, info = build_graph(base, index.metric, R, knn_k, nav_L, knn_fn, search_fn, alpha,
                                 passes, nthreads, log)

@@ -15,7 +15,8 @@ METRIC_L2 = 0
 METRIC_INNER_PRODUCT = 1
 
 STATS_DTYPE = np.dtype([("ndc", np.uint32), ("hops", np.uint32),
-                        ("lossy", np.uint32), ("reserved", np.uint32)])
+                        ("lossy", np.uint32), ("reserved", np.uint32),
+                        ("t_start", np.uint64), ("t_end", np.uint64)])
 
 
 def dim_padded(dim):
@@ -85,6 +86,11 @@ class Index:
                                 ctypes.byref(e), ctypes.byref(m)))
         return dict(dim=d.value, num=n.value, r_pad=r.value, ep=e.value, metric=m.value)
 
+    def wall_clock_khz(self):
+        k = ctypes.c_uint32()
+        check(lib.ss_wall_clock_khz(self._h, ctypes.byref(k)))
+        return k.value
+
     # -- graph ------------------------------------------------------------
     def load_graph(self, path):
         check(lib.ss_load_graph(self._h, str(path).encode()))
@@ -133,6 +139,21 @@ class Index:
                             ctypes.byref(p), uptr(ids), fptr(dists),
                             stats.ctypes.data_as(ctypes.POINTER(QueryStats))))
         return ids, dists, stats
+
+    def search_path(self, queries, K, L, path_cap, visited_slots=0):
+        """Search with one K and L, also returning the expanded node ids of
+        every query in order ([nq][path_cap], SS_NONE-padded)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, K), dtype=np.uint32)
+        dists = np.empty((nq, K), dtype=np.float32)
+        stats = np.zeros(nq, dtype=STATS_DTYPE)
+        path = np.empty((nq, path_cap), dtype=np.uint32)
+        p = _params(visited_slots=visited_slots)
+        check(lib.ss_search_path(self._h, fptr(q), nq, int(K), int(L), int(K), ctypes.byref(p),
+                                 int(path_cap), uptr(ids), fptr(dists),
+                                 stats.ctypes.data_as(ctypes.POINTER(QueryStats)), uptr(path)))
+        return ids, dists, stats, path
 
     def search_device(self, d_queries, nq, k_max, l_max, k_stride, d_out_ids, d_out_dists,
                       d_stats=0, d_K=0, d_L=0, stream=0, mode=_lib.SS_MODE_BATCHED,
