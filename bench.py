@@ -109,10 +109,17 @@ def main():
 
     if args.profile:
         L = args.L or 64
-        for _ in range(args.steps):
-            launch(L)
+        with torch.cuda.stream(stream):
+            for _ in range(args.steps):
+                launch(L)
+            # calibration for HBM counters: a plain copy of a known size
+            # (512 MiB read + 512 MiB written per launch)
+            src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(5):
+                dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"profile: {args.steps} launches at L={L} done")
+        log(f"profile: {args.steps} launches at L={L} and 5 copies of 512 MiB done")
         return
 
     t0 = time.time()
