@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--profile", action="store_true",
                     help="only build, then run --steps batched launches (for rocprofv3)")
+    ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000",
+                    help="Poisson arrival rates (QPS) of the serving sweep; empty to skip")
+    ap.add_argument("--serve-requests", type=int, default=20000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -195,6 +198,11 @@ def main():
     plain_qps = npq / (time.perf_counter() - t)
     log(f"plain per-query launches: {plain_qps:.0f} QPS over {npq} queries")
 
+    serving = None
+    if args.serve_rates:
+        serving = serve_sweep(ix, queries, gt, w, [float(r) for r in args.serve_rates.split(",")],
+                              args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(base, adj, ep, queries, gt, K, L, w.metric, args.cpu_seconds)
@@ -217,6 +225,7 @@ def main():
                    "mean_ndc": round(float(st["ndc"].mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1)},
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
+        "serving": serving,
         "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
                        "service_p99": round(metrics.percentile(service_ms, 0.99), 4),
                        "batch_completion_p50": round(metrics.percentile(done_ms, 0.5), 4),
@@ -230,6 +239,35 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def serve_sweep(ix, queries, gt10, w, rates, nreq):
+    """Online serving (the C4 workload): Poisson arrivals at each rate into
+    the resident kernel, K = 10 (80%) or 100 (20%), L uniform in 64..512;
+    latency = completion seen by the host minus arrival."""
+    from scattersearch_amd import metrics, serve
+    rng = np.random.default_rng(20261207)
+    qi = np.arange(nreq) % queries.shape[0]
+    K = np.where(rng.random(nreq) < 0.8, 10, 100).astype(np.uint32)
+    L = rng.integers(64, 513, size=nreq).astype(np.uint32)
+    Q = queries[qi]
+    out = []
+    for r in rates:
+        arr = serve.poisson_arrivals(nreq, r, seed=int(r))
+        t = time.perf_counter()
+        ids, ds, lat, st = serve.replay(ix, Q, K, L, arr, k_max=100, l_max=512, idle_timeout_ms=2000)
+        wall = time.perf_counter() - t
+        span_s = (arr[-1] + lat[-1]) * 1e-6
+        rec = metrics.recall(ids[:, :10], gt10[qi], 10)
+        row = {"rate_qps": r, "requests": nreq, "achieved_qps": round(nreq / max(span_s, 1e-9), 1),
+               "p50_ms": round(metrics.percentile(lat, 0.5) / 1e3, 4),
+               "p99_ms": round(metrics.percentile(lat, 0.99) / 1e3, 4),
+               "mean_ms": round(float(lat.mean()) / 1e3, 4), "recall@10": round(rec, 4),
+               "mean_ndc": round(float(st["ndc"].mean()), 1), "wall_s": round(wall, 2)}
+        out.append(row)
+        log(f"serve {row}")
+    return {"workload": "resident kernel, Poisson arrivals, K=10 (80%) / 100 (20%), L uniform 64..512",
+            "rates": out}
 
 
 def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):

@@ -56,7 +56,13 @@ typedef struct ss_search_params {
     uint32_t blocks;          /* persistent grid size; 0 = enough to fill the GPU */
     uint32_t use_graph;       /* 1: capture the batched launch in a hipGraph
                                  and replay it for identical shapes */
-    uint32_t reserved[3];
+    uint32_t expand_width;    /* pool entries expanded per step: 1 (default)
+                                 reproduces SearchArraySimulation exactly; 2
+                                 or 4 expand the first 2/4 unexplored entries
+                                 together (ModifiedDeltaStepping,
+                                 src/annsearch.cpp:1586-1731): fewer dependent
+                                 steps, recall instead of exact parity */
+    uint32_t reserved[2];
 } ss_search_params;
 
 /* Per-query counters written by every search. */
@@ -153,6 +159,53 @@ int ss_bruteforce_device(ss_index *index, const float *d_queries, uint32_t nq, u
 /* k-NN graph of the base itself (each point's k nearest other points),
  * written to host arrays [num][k]. */
 int ss_knn_graph(ss_index *index, uint32_t k, uint32_t *out_ids, float *out_dists);
+
+/* ---------------------------------------------------------------------- */
+/* Continuous-batching serving (tests/online_parallel_search.cpp:78-199,    */
+/* tests/online_pipeline_search.cpp:136-172).  A kernel stays resident on   */
+/* the GPU with the graph and per-query state; requests submitted while     */
+/* others run are admitted as wavefronts free up and overlap on the device. */
+/* One thread may submit while another polls.                               */
+/* ---------------------------------------------------------------------- */
+typedef struct ss_server ss_server;
+
+typedef struct ss_serve_config {
+    uint32_t k_max;           /* largest K a request may ask for */
+    uint32_t l_max;           /* largest L (sizes each wavefront's LDS pool) */
+    uint32_t ring_slots;      /* most requests in flight (power of two); 0 = 16384 */
+    uint32_t visited_slots;   /* per-wavefront visited hash; 0 = tuned */
+    uint32_t waves_per_block; /* 0 = tuned */
+    uint32_t blocks;          /* resident grid; 0 = fill the GPU */
+    uint32_t idle_timeout_ms; /* the resident kernel exits after this long
+                                 without requests and restarts on the next
+                                 one; 0 = 2000 */
+    uint32_t reserved;
+} ss_serve_config;
+
+int ss_serve_start(ss_index *index, const ss_serve_config *config, ss_server **out);
+
+/* Queue one query ([dim] floats, host).  Returns SS_EBUSY when ring_slots
+ * requests are in flight; *ticket numbers requests 0, 1, 2, ... */
+int ss_submit(ss_server *server, const float *query, uint32_t K, uint32_t L, uint64_t *ticket);
+
+/* Collect up to max_results finished requests in completion order: tickets,
+ * ids/dists [n][k_max] (entries past the request's K are SS_NONE / +inf)
+ * and stats may be NULL except tickets. */
+int ss_poll(ss_server *server, uint32_t max_results, uint64_t *tickets, uint32_t *ids,
+            float *dists, ss_query_stats *stats, uint32_t *n_out);
+
+/* Stop the resident kernel and free the server; unpolled results are lost. */
+int ss_serve_stop(ss_server *server);
+
+/* Open-loop load test: starts a server, submits query i at arrival_us[i]
+ * microseconds after the start from one thread while another polls, and
+ * returns each result with its latency (completion seen by the poller minus
+ * arrival_us[i], the definition of tests/online_parallel_search.cpp:154-158)
+ * in out_ids/out_dists [nq][k_stride] and latency_us [nq]. */
+int ss_serve_replay(ss_index *index, const ss_serve_config *config, const float *queries,
+                    uint32_t nq, const uint32_t *K, const uint32_t *L, const double *arrival_us,
+                    uint32_t k_stride, uint32_t *out_ids, float *out_dists, double *latency_us,
+                    ss_query_stats *stats);
 
 /* ---------------------------------------------------------------------- */
 /* Host-side files and graph construction (no GPU needed).                  */

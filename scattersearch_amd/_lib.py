@@ -33,7 +33,8 @@ class SearchParams(ctypes.Structure):
         ("visited_slots", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("expand_width", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -45,6 +46,19 @@ class QueryStats(ctypes.Structure):
         ("reserved", ctypes.c_uint32),
         ("t_start", ctypes.c_uint64),
         ("t_end", ctypes.c_uint64),
+    ]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("k_max", ctypes.c_uint32),
+        ("l_max", ctypes.c_uint32),
+        ("ring_slots", ctypes.c_uint32),
+        ("visited_slots", ctypes.c_uint32),
+        ("waves_per_block", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32),
+        ("idle_timeout_ms", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -101,6 +115,14 @@ _SIGS = {
     "ss_bruteforce": (cint, [vp, f32p, u32, u32, u32p, u32p, f32p]),
     "ss_bruteforce_device": (cint, [vp, vp, u32, u32, cint, u32, vp, vp, vp]),
     "ss_knn_graph": (cint, [vp, u32, u32p, f32p]),
+    "ss_serve_start": (cint, [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]),
+    "ss_submit": (cint, [vp, f32p, u32, u32, ctypes.POINTER(ctypes.c_uint64)]),
+    "ss_poll": (cint, [vp, u32, ctypes.POINTER(ctypes.c_uint64), u32p, f32p,
+                       ctypes.POINTER(QueryStats), u32p]),
+    "ss_serve_stop": (cint, [vp]),
+    "ss_serve_replay": (cint, [vp, ctypes.POINTER(ServeConfig), f32p, u32, u32p, u32p,
+                               ctypes.POINTER(ctypes.c_double), u32, u32p, f32p,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(QueryStats)]),
     "ss_fbin_header": (cint, [cstr, u32p, u32p]),
     "ss_fbin_read": (cint, [cstr, f32p, u32, u32]),
     "ss_fbin_write": (cint, [cstr, f32p, u32, u32]),

@@ -394,8 +394,9 @@ int ss_load_graph(ss_index *ix, const char *path)
 }
 
 // ---------------------------------------------------------------------------
-// Launch shape.  The visited hash gets about 64 slots per pool entry (room
-// for the candidates of the expansions a pool of L needs), the waves per
+// Launch shape.  The visited hash holds the ids whose distances a query
+// computes (about 13 per expanded node at R=32); 2048 slots keep the load
+// factor under one half up to L=64, larger pools get more.  The waves per
 // block and the grid fill the LDS of every CU.
 // ---------------------------------------------------------------------------
 struct Shape {
@@ -413,12 +414,18 @@ static uint32_t next_pow2(uint32_t x)
 
 static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const ss_search_params *p, Shape *s)
 {
-    uint32_t slots = p && p->visited_slots ? p->visited_slots : std::min<uint32_t>(8192, std::max<uint32_t>(1024, next_pow2(l_max * 64)));
+    const uint32_t ew = p && p->expand_width ? p->expand_width : 1;
+    if (ew != 1 && ew != 2 && ew != 4) {
+        set_error("expand_width must be 1, 2 or 4");
+        return SS_EINVAL;
+    }
+    uint32_t slots = p && p->visited_slots ? p->visited_slots
+                                           : std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(l_max * 16)));
     if (slots < 256 || slots > 32768 || (slots & (slots - 1))) {
         set_error("visited_slots must be a power of two in 256..32768");
         return SS_EINVAL;
     }
-    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots);
+    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots, ew);
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
     const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
     if (waves_cu == 0) {
@@ -440,6 +447,58 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
     s->lds_wave = per_wave;
     return SS_OK;
 }
+
+}  // extern "C"
+
+namespace ss {
+struct IndexView {
+    const float *d_base;
+    const uint32_t *d_adj;
+    uint32_t dim, dim_pad, num, r_pad, ep;
+    int metric, num_cus;
+    size_t lds_per_cu;
+    bool has_graph;
+};
+
+int index_view(const ss_index *ix, IndexView *v)
+{
+    if (!ix || !v) {
+        set_error("NULL index");
+        return SS_EINVAL;
+    }
+    v->d_base = ix->d_base;
+    v->d_adj = ix->d_adj;
+    v->dim = ix->dim;
+    v->dim_pad = ix->dim_pad;
+    v->num = ix->num;
+    v->r_pad = ix->r_pad;
+    v->ep = ix->ep;
+    v->metric = ix->metric;
+    v->num_cus = ix->num_cus;
+    v->lds_per_cu = ix->lds_per_cu;
+    v->has_graph = ix->has_graph;
+    return SS_OK;
+}
+
+int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,
+              uint32_t blocks, uint32_t *wpb, uint32_t *slots, uint32_t *blocks_out)
+{
+    ss_search_params p{};
+    p.visited_slots = visited_slots;
+    p.waves_per_block = waves_per_block;
+    p.blocks = blocks;
+    Shape sh;
+    int rc = choose_shape(ix, l_max, nq, &p, &sh);
+    if (rc)
+        return rc;
+    *wpb = sh.wpb;
+    *slots = sh.slots;
+    *blocks_out = sh.blocks;
+    return SS_OK;
+}
+}  // namespace ss
+
+extern "C" {
 
 static uint32_t log2u(uint32_t x)
 {
@@ -500,6 +559,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.stats = d_stats;
     a.path = d_path;
     a.path_cap = d_path ? path_cap : 0;
+    a.expand_width = p && p->expand_width ? p->expand_width : 1;
     hipError_t e;
     if (mode == SS_MODE_PER_QUERY) {
         a.queue = nullptr;

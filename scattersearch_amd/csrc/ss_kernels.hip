@@ -181,16 +181,13 @@ __device__ __forceinline__ bool visit(uint32_t *tab, uint32_t shift, uint32_t ma
 {
     uint32_t s = (id * 0x9E3779B1u) >> shift;
     for (uint32_t p = 0; p < kMaxProbe; p++) {
-        const uint32_t cur = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (cur == id)
+        // One LDS round trip per probe: claim the slot if it is empty, and
+        // learn what it holds otherwise.
+        const uint32_t old = atomicCAS(&tab[s], kNone, id);
+        if (old == kNone)
+            return true;
+        if (old == id)
             return false;
-        if (cur == kNone) {
-            const uint32_t old = atomicCAS(&tab[s], kNone, id);
-            if (old == kNone)
-                return true;
-            if (old == id)
-                return false;
-        }
         s = (s + 1u) & mask;
     }
     lossy++;
@@ -208,12 +205,13 @@ struct WaveLds {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t wave_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots)
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+                                                 uint32_t ew = 1)
 {
-    return align16(dim_pad * 4u) + align16(l_max * 4u) * 2 + 64 * 4 * 2 + align16(hash_slots * 4u);
+    return align16(dim_pad * 4u) + align16(l_max * 4u) * 2 + 64 * 4 * 2 * ew + align16(hash_slots * 4u);
 }
 
-__device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_max)
+__device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_max, uint32_t ew = 1)
 {
     WaveLds w;
     w.qry = reinterpret_cast<float *>(p);
@@ -223,9 +221,9 @@ __device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_m
     w.pool_d = reinterpret_cast<float *>(p);
     p += align16(l_max * 4u);
     w.cand_id = reinterpret_cast<uint32_t *>(p);
-    p += 64 * 4;
+    p += 64 * 4 * ew;
     w.cand_d = reinterpret_cast<float *>(p);
-    p += 64 * 4;
+    p += 64 * 4 * ew;
     w.hash = reinterpret_cast<uint32_t *>(p);
     return w;
 }
@@ -249,6 +247,100 @@ __device__ __forceinline__ void candidate_distances(const SearchArgs &a, const W
     }
 }
 
+// Per-query state shared by the search variants.
+struct QueryState {
+    uint32_t K, L, t, ndc, hops, lossy;
+    uint64_t t_start;
+    uint32_t *out_ids;
+    float *out_d;
+};
+
+// Validates K and L, loads the query into LDS, clears the visited hash and
+// seeds the pool with the first min(L, deg(ep)) neighbours of the entry
+// point, marked visited (ep itself is neither pooled nor marked),
+// src/annsearch.cpp:185-215.  Returns false (after writing an empty result)
+// for a request the host should have rejected.
+template <int NBLK, int TAILI, int METRIC>
+__device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st)
+{
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    st.t_start = (uint64_t)wall_clock64();
+    st.K = uniform(a.Ks ? a.Ks[q] : a.k_default);
+    st.L = uniform(a.Ls ? a.Ls[q] : a.l_default);
+    st.out_ids = a.out_ids + (size_t)q * a.k_stride;
+    st.out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
+    st.lossy = 0;
+    st.hops = 0;
+    if (st.K == 0 || st.L == 0 || st.K > st.L || st.L > a.l_max || st.K > a.k_stride) {
+        for (uint32_t i = lane; i < a.k_stride; i += 64) {
+            st.out_ids[i] = kNone;
+            if (st.out_d)
+                st.out_d[i] = __builtin_inff();
+        }
+        if (a.stats && lane == 0)
+            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, st.t_start, (uint64_t)wall_clock64()};
+        return false;
+    }
+    const float4 *qsrc = reinterpret_cast<const float4 *>(a.queries + (size_t)q * a.dim_pad);
+    float4 *qdst = reinterpret_cast<float4 *>(w.qry);
+    for (uint32_t i = lane; i < a.dim_pad / 4; i += 64)
+        qdst[i] = qsrc[i];
+    uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
+    for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
+        h4[i] = make_uint4(kNone, kNone, kNone, kNone);
+    wave_sync();
+    const uint32_t nb = lane < a.r_pad ? a.adj[(size_t)a.ep * a.r_pad + lane] : kNone;
+    const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
+    const uint32_t t = deg < st.L ? deg : st.L;
+    if (lane < t) {
+        visit(w.hash, a.hash_shift, a.hash_slots - 1u, nb, st.lossy);
+        w.cand_id[lane] = nb;
+    }
+    wave_sync();
+    candidate_distances<NBLK, TAILI, METRIC>(a, w, t, nblk_rt, tail_rt);
+    wave_sync();
+    const float di = lane < t ? w.cand_d[lane] : __builtin_inff();
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < t; j++) {
+        const float dj = readlane_f(di, j);
+        rank += (dj < di || (dj == di && j < lane)) ? 1u : 0u;
+    }
+    wave_sync();
+    if (lane < t) {
+        w.pool_id[rank] = nb;
+        w.pool_d[rank] = di;
+    }
+    st.t = t;
+    st.ndc = t;
+    wave_sync();
+    return true;
+}
+
+// indices[i] = retset[i].id, i < K (src/annsearch.cpp:254-257); distances
+// as the pipeline form returns them (:319-323); counters and timestamps.
+__device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w, uint32_t q, const QueryState &st)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t i = lane; i < a.k_stride; i += 64) {
+        const bool ok = i < st.K && i < st.t;
+        st.out_ids[i] = ok ? (w.pool_id[i] & ~kExplored) : kNone;
+        if (st.out_d)
+            st.out_d[i] = ok ? w.pool_d[i] : __builtin_inff();
+    }
+    if (a.path)
+        for (uint32_t i = st.hops + lane; i < a.path_cap; i += 64)
+            a.path[(size_t)q * a.path_cap + i] = kNone;
+    uint32_t lsum = st.lossy;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        lsum += __shfl_xor(lsum, off, 64);
+    if (a.stats && lane == 0)
+        a.stats[q] = ss_query_stats{st.ndc, st.hops, lsum, 0u, st.t_start, (uint64_t)wall_clock64()};
+    wave_sync();
+}
+
 // One query: src/annsearch.cpp:182-259 as a sequence of whole-row
 // expansions.  For the expansion of pool entry k with pool P (size t, tail
 // distance tau = P[t-1].dist), let A be the unvisited neighbours with
@@ -258,107 +350,51 @@ __device__ __forceinline__ void candidate_distances(const SearchArgs &a, const W
 // the stable merge of P and A (P first on equal distances, A in row order),
 // and its jump target nk (:244-252) is the merged position of the first
 // smallest element of A.  Every entry before the cursor is explored, so the
-// next node is the first unexplored pool entry.
+// next node is the first unexplored entry of the merged pool.
 template <int NBLK, int TAILI, int METRIC>
 __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
 {
-    const uint64_t t_start = (uint64_t)wall_clock64();
+    QueryState st;
+    if (!query_begin<NBLK, TAILI, METRIC>(a, w, q, st))
+        return;
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
-    const uint32_t K = uniform(a.Ks ? a.Ks[q] : a.k_default);
-    const uint32_t L = uniform(a.Ls ? a.Ls[q] : a.l_default);
-    uint32_t *out_ids = a.out_ids + (size_t)q * a.k_stride;
-    float *out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
-
-    if (K == 0 || L == 0 || K > L || L > a.l_max || K > a.k_stride) {
-        // Rejected by the host already; fill the row so nothing is stale.
-        for (uint32_t i = lane; i < a.k_stride; i += 64) {
-            out_ids[i] = kNone;
-            if (out_d)
-                out_d[i] = __builtin_inff();
-        }
-        if (a.stats && lane == 0)
-            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, t_start, (uint64_t)wall_clock64()};
-        return;
-    }
-
-    // Query vector and an empty visited hash.
-    const float4 *qsrc = reinterpret_cast<const float4 *>(a.queries + (size_t)q * a.dim_pad);
-    float4 *qdst = reinterpret_cast<float4 *>(w.qry);
-    for (uint32_t i = lane; i < a.dim_pad / 4; i += 64)
-        qdst[i] = qsrc[i];
-    uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
-    for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
-        h4[i] = make_uint4(kNone, kNone, kNone, kNone);
     const uint32_t hmask = a.hash_slots - 1u;
-    uint32_t lossy = 0;
-    wave_sync();
+    const uint32_t L = st.L;
 
-    // Seed pool: the first min(L, deg(ep)) neighbours of the entry point,
-    // marked visited (ep itself is neither pooled nor marked), :185-215.
-    uint32_t nb = lane < a.r_pad ? a.adj[(size_t)a.ep * a.r_pad + lane] : kNone;
-    const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
-    uint32_t t = deg < L ? deg : L;
-    if (lane < t) {
-        visit(w.hash, a.hash_shift, hmask, nb, lossy);
-        w.cand_id[lane] = nb;
+    // Expansion loop.  k is the pool position of the node to expand and nb
+    // already holds its adjacency row: the next node is decided from
+    // registers before the merge is written, and its row load is issued
+    // then, so the load overlaps the merge.
+    uint32_t t = st.t;
+    uint32_t k = 0;
+    bool have = t > 0;
+    uint32_t nb = kNone;
+    if (have) {
+        const uint32_t n0 = uniform(w.pool_id[0]);
+        nb = lane < a.r_pad ? a.adj[(size_t)n0 * a.r_pad + lane] : kNone;
     }
-    wave_sync();
-    candidate_distances<NBLK, TAILI, METRIC>(a, w, t, nblk_rt, tail_rt);
-    wave_sync();
-    {
-        const float di = lane < t ? w.cand_d[lane] : __builtin_inff();
-        uint32_t rank = 0;
-        for (uint32_t j = 0; j < t; j++) {
-            const float dj = readlane_f(di, j);
-            rank += (dj < di || (dj == di && j < lane)) ? 1u : 0u;
-        }
-        wave_sync();
-        if (lane < t) {
-            w.pool_id[rank] = nb;
-            w.pool_d[rank] = di;
-        }
-    }
-    uint32_t ndc = t, hops = 0;
-    wave_sync();
-
-    uint32_t start = 0;
-    while (true) {
-        // Next node: first unexplored entry at or after the cursor.
-        int found = -1;
-        for (uint32_t c0 = start & ~63u; c0 < t; c0 += 64) {
-            const uint32_t i = c0 + lane;
-            const bool un = i >= start && i < t && !(w.pool_id[i] & kExplored);
-            const uint64_t m = ballot(un);
-            if (m) {
-                found = (int)(c0 + (uint32_t)__ffsll((unsigned long long)m) - 1u);
-                break;
-            }
-        }
-        if (found < 0)
-            break;
-        const uint32_t k = uniform((uint32_t)found);
+    while (have) {
         const uint32_t n = uniform(w.pool_id[k]);
         const float tail = readlane_f(w.pool_d[t - 1], 0);
         wave_sync();
         if (lane == 0) {
             w.pool_id[k] = n | kExplored;
-            if (a.path && hops < a.path_cap)
-                a.path[(size_t)q * a.path_cap + hops] = n;
+            if (a.path && st.hops < a.path_cap)
+                a.path[(size_t)q * a.path_cap + st.hops] = n;
         }
-        hops++;
+        st.hops++;
 
-        // Neighbour row (one coalesced load), visited test, compaction.
-        nb = lane < a.r_pad ? a.adj[(size_t)n * a.r_pad + lane] : kNone;
-        const bool fresh = nb != kNone && visit(w.hash, a.hash_shift, hmask, nb, lossy);
+        // Visited test of the whole row, compaction of the fresh ids.
+        const bool fresh = nb != kNone && visit(w.hash, a.hash_shift, hmask, nb, st.lossy);
         const uint64_t fm = ballot(fresh);
         const uint32_t nc = (uint32_t)__popcll(fm);
         if (fresh)
             w.cand_id[lanes_below(fm)] = nb;
         wave_sync();
         candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
-        ndc += nc;
+        st.ndc += nc;
         wave_sync();
 
         float cd = __builtin_inff();
@@ -385,9 +421,28 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
             }
         }
 
+        // First unexplored entry after k in the pool as it is now (every entry
+        // up to k is explored).
+        int onext = -1;
+        uint32_t oid = kNone;
+        float od = __builtin_inff();
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; c++) {
+            if ((uint32_t)c < nch && onext < 0) {
+                const uint32_t i = (uint32_t)c * 64u + lane;
+                const uint64_t m = ballot(i > k && i < t && !(pid[c] & kExplored));
+                if (m) {
+                    const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    onext = c * 64 + (int)l;
+                    oid = readlane_u(pid[c], l);
+                    od = readlane_f(pdv[c], l);
+                }
+            }
+        }
+
         // After a lossy insert a pool member can come back as a candidate
         // (same id, same distance): drop it.
-        if (ballot(lossy != 0)) {
+        if (ballot(st.lossy != 0)) {
             uint64_t it = ballot(inA);
             while (it) {
                 const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
@@ -403,8 +458,12 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         }
 
         const uint64_t am = ballot(inA);
-        if (am == 0) {  // nothing admitted: nk = L, continue at k+1 (:249-252)
-            start = k + 1;
+        if (am == 0) {  // nothing admitted: nk = L, the cursor moves on (:249-252)
+            have = onext >= 0;
+            if (have) {
+                k = (uint32_t)onext;
+                nb = lane < a.r_pad ? a.adj[(size_t)oid * a.r_pad + lane] : kNone;
+            }
             continue;
         }
         const uint32_t na = (uint32_t)__popcll(am);
@@ -433,7 +492,22 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         posA += rankA;
         const uint64_t mm = ballot(inA && rankA == 0);
         const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)mm) - 1u;
-        const uint32_t nk = readlane_u(posA, jmin);
+        const uint32_t nk = readlane_u(posA, jmin);  // :244-245
+        const float dmin = readlane_f(cd, jmin);
+
+        // Next node: the first unexplored entry of the merged pool is either
+        // the old one found above (it keeps position onext when no admitted
+        // candidate is smaller, P first on ties) or the smallest admitted
+        // candidate at position nk; both lie inside the first L.
+        uint32_t next_id;
+        if (onext >= 0 && od <= dmin) {
+            k = (uint32_t)onext;
+            next_id = oid;
+        } else {
+            k = nk;
+            next_id = readlane_u(cid, jmin);
+        }
+        nb = lane < a.r_pad ? a.adj[(size_t)next_id * a.r_pad + lane] : kNone;
 
         wave_sync();
 #pragma unroll
@@ -452,56 +526,348 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
             w.pool_d[posA] = cd;
         }
         t = t + na < L ? t + na : L;
-        start = nk <= k ? nk : k + 1;
         wave_sync();
     }
+    st.t = t;
 
-    // indices[i] = retset[i].id, i < K (:254-257); distances as the
-    // pipeline form returns them (:319-323).
-    for (uint32_t i = lane; i < a.k_stride; i += 64) {
-        const bool ok = i < K && i < t;
-        out_ids[i] = ok ? (w.pool_id[i] & ~kExplored) : kNone;
-        if (out_d)
-            out_d[i] = ok ? w.pool_d[i] : __builtin_inff();
-    }
-    if (a.path)
-        for (uint32_t i = hops + lane; i < a.path_cap; i += 64)
-            a.path[(size_t)q * a.path_cap + i] = kNone;
-    uint32_t lsum = lossy;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        lsum += __shfl_xor(lsum, off, 64);
-    if (a.stats && lane == 0)
-        a.stats[q] = ss_query_stats{ndc, hops, lsum, 0u, t_start, (uint64_t)wall_clock64()};
-    wave_sync();
+    query_end(a, w, q, st);
 }
 
-template <int NBLK, int TAILI, int METRIC>
+// Wider expansion (the ModifiedDeltaStepping idea, src/annsearch.cpp:
+// 1586-1731): each step expands the first EW unexplored pool entries at
+// once - EW adjacency rows loaded together, all their fresh neighbours'
+// distances gathered together, one merge against the tail at the start of
+// the step.  Fewer dependent steps per query; the expansion order is relaxed,
+// so this variant is judged by recall, not by parity.
+template <int NBLK, int TAILI, int METRIC, int EW>
+__device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
+{
+    QueryState st;
+    if (!query_begin<NBLK, TAILI, METRIC>(a, w, q, st))
+        return;
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    const uint32_t hmask = a.hash_slots - 1u;
+    const uint32_t L = st.L;
+    uint32_t t = st.t;
+    while (true) {
+        // The first EW unexplored entries.
+        uint32_t ks[EW], ns[EW];
+        uint32_t nw = 0;
+        for (uint32_t c0 = 0; c0 < t && nw < EW; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            uint64_t m = ballot(i < t && !(w.pool_id[i] & kExplored));
+#pragma unroll
+            for (int r = 0; r < EW; r++) {
+                if ((uint32_t)r >= nw && m) {
+                    ks[r] = c0 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1ull;
+                    nw = (uint32_t)r + 1;
+                }
+            }
+        }
+        if (nw == 0)
+            break;
+        const float tail = readlane_f(w.pool_d[t - 1], 0);
+#pragma unroll
+        for (int r = 0; r < EW; r++)
+            ns[r] = (uint32_t)r < nw ? uniform(w.pool_id[ks[r]]) : kNone;
+        uint32_t nb[EW];
+#pragma unroll
+        for (int r = 0; r < EW; r++)
+            nb[r] = ((uint32_t)r < nw && lane < a.r_pad) ? a.adj[(size_t)ns[r] * a.r_pad + lane] : kNone;
+        wave_sync();
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < EW; r++) {
+                if ((uint32_t)r < nw) {
+                    w.pool_id[ks[r]] = ns[r] | kExplored;
+                    if (a.path && st.hops + r < a.path_cap)
+                        a.path[(size_t)q * a.path_cap + st.hops + r] = ns[r];
+                }
+            }
+        }
+        st.hops += nw;
+
+        // Visited test row by row (a node in two rows counts once), candidates
+        // compacted in (row, column) order.
+        uint32_t nc = 0;
+#pragma unroll
+        for (int r = 0; r < EW; r++) {
+            const bool fresh = nb[r] != kNone && visit(w.hash, a.hash_shift, hmask, nb[r], st.lossy);
+            const uint64_t fm = ballot(fresh);
+            if (fresh)
+                w.cand_id[nc + lanes_below(fm)] = nb[r];
+            nc += (uint32_t)__popcll(fm);
+        }
+        wave_sync();
+        candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
+        st.ndc += nc;
+        wave_sync();
+
+        float cd[EW];
+        uint32_t cid[EW];
+        bool inA[EW];
+#pragma unroll
+        for (int s = 0; s < EW; s++) {
+            const uint32_t c = (uint32_t)s * 64u + lane;
+            cd[s] = c < nc ? w.cand_d[c] : __builtin_inff();
+            cid[s] = c < nc ? w.cand_id[c] : kNone;
+            inA[s] = c < nc && cd[s] < tail;
+        }
+        const uint32_t nch = (t + 63u) >> 6;
+        uint32_t pid[kMaxChunks];
+        float pdv[kMaxChunks];
+        uint32_t cnt[kMaxChunks];
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; c++) {
+            pid[c] = kNone;
+            pdv[c] = __builtin_inff();
+            cnt[c] = 0;
+            const uint32_t i = (uint32_t)c * 64u + lane;
+            if ((uint32_t)c < nch && i < t) {
+                pid[c] = w.pool_id[i];
+                pdv[c] = w.pool_d[i];
+            }
+        }
+        if (ballot(st.lossy != 0)) {
+#pragma unroll
+            for (int s = 0; s < EW; s++) {
+                uint64_t it = ballot(inA[s]);
+                while (it) {
+                    const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
+                    it &= it - 1ull;
+                    const uint32_t x = readlane_u(cid[s], j);
+                    bool hit = false;
+#pragma unroll
+                    for (int c = 0; c < kMaxChunks; c++)
+                        hit |= (uint32_t)c < nch && (pid[c] & ~kExplored) == x && pid[c] != kNone;
+                    if (ballot(hit) && lane == j)
+                        inA[s] = false;
+                }
+            }
+        }
+        uint64_t am[EW];
+        uint32_t na = 0;
+#pragma unroll
+        for (int s = 0; s < EW; s++) {
+            am[s] = ballot(inA[s]);
+            na += (uint32_t)__popcll(am[s]);
+        }
+        if (na == 0)
+            continue;
+        uint32_t rankA[EW], posA[EW];
+#pragma unroll
+        for (int s = 0; s < EW; s++)
+            rankA[s] = posA[s] = 0;
+#pragma unroll
+        for (int sj = 0; sj < EW; sj++) {
+            uint64_t it = am[sj];
+            while (it) {
+                const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
+                it &= it - 1ull;
+                const float dj = readlane_f(cd[sj], j);
+                const uint32_t gj = (uint32_t)sj * 64u + j;
+#pragma unroll
+                for (int s = 0; s < EW; s++) {
+                    const uint32_t gs = (uint32_t)s * 64u + lane;
+                    rankA[s] += (inA[s] && (dj < cd[s] || (dj == cd[s] && gj < gs))) ? 1u : 0u;
+                }
+                uint32_t le = 0;
+#pragma unroll
+                for (int c = 0; c < kMaxChunks; c++) {
+                    if ((uint32_t)c < nch) {
+                        cnt[c] += dj < pdv[c] ? 1u : 0u;
+                        const uint32_t i = (uint32_t)c * 64u + lane;
+                        le += (uint32_t)__popcll(ballot(i < t && pdv[c] <= dj));
+                    }
+                }
+                if (lane == j)
+                    posA[sj] = le;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; c++) {
+            const uint32_t i = (uint32_t)c * 64u + lane;
+            if ((uint32_t)c < nch && i < t) {
+                const uint32_t np = i + cnt[c];
+                if (np < L) {
+                    w.pool_id[np] = pid[c];
+                    w.pool_d[np] = pdv[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < EW; s++) {
+            const uint32_t pos = posA[s] + rankA[s];
+            if (inA[s] && pos < L) {
+                w.pool_id[pos] = cid[s];
+                w.pool_d[pos] = cd[s];
+            }
+        }
+        t = t + na < L ? t + na : L;
+        wave_sync();
+    }
+    st.t = t;
+    query_end(a, w, q, st);
+}
+
+template <int NBLK, int TAILI, int METRIC, int EW>
 __global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t waves = blockDim.x >> 6;
-    const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots);
-    const WaveLds w = carve(smem + per_wave * wave, a.dim_pad, a.l_max);
-    if (a.queue) {
-        while (true) {
-            uint32_t q = 0;
+    const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, EW);
+    const WaveLds w = carve(smem + per_wave * wave, a.dim_pad, a.l_max, EW);
+    // One call site for the search body (two would keep it out of line).
+    bool first = true;
+    while (true) {
+        uint32_t q;
+        if (a.queue) {
+            q = 0;
             if (lane_id() == 0)
                 q = atomicAdd(a.queue, 1u);
             q = uniform(q);
-            if (q >= a.nq)
+        } else {
+            if (!first)
                 break;
-            search_one<NBLK, TAILI, METRIC>(a, w, a.q_first + q);
+            q = blockIdx.x * waves + wave;
         }
-    } else {
-        const uint32_t q = blockIdx.x * waves + wave;
-        if (q < a.nq)
+        first = false;
+        if (q >= a.nq)
+            break;
+        if constexpr (EW == 1)
             search_one<NBLK, TAILI, METRIC>(a, w, a.q_first + q);
+        else
+            search_wide<NBLK, TAILI, METRIC, EW>(a, w, a.q_first + q);
     }
 }
 
 __global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
+
+// ---------------------------------------------------------------------------
+// Resident serving kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Copies published requests into device slots and publishes them to the
+// workers; exits on the host's stop flag or after idle_ticks without a new
+// request, and then raises the device stop flag every worker watches.
+__device__ void serve_dispatch(const ServeArgs &sa)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t mask = sa.slots - 1u;
+    const uint32_t nv4 = sa.s.dim_pad / 4;
+    uint64_t copied = uniform64(__hip_atomic_load(&sa.d_ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    uint64_t last = (uint64_t)wall_clock64();
+    while (true) {
+        uint32_t stop = 0;
+        uint64_t head = 0;
+        if (lane == 0) {
+            stop = __hip_atomic_load(sa.h_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            head = __hip_atomic_load(sa.h_head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        stop = uniform(stop);
+        head = uniform64(head);
+        if (lane == 0)
+            __hip_atomic_store(&sa.d_ctl[3], (uint64_t)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (stop)
+            break;
+        if (head > copied) {
+            const uint64_t end = head - copied > 64 ? copied + 64 : head;
+            for (uint64_t i = copied; i < end; i++) {
+                const uint32_t slot = (uint32_t)i & mask;
+                const float4 *src = reinterpret_cast<const float4 *>(sa.h_q + (size_t)slot * sa.s.dim_pad);
+                float4 *dst = reinterpret_cast<float4 *>(sa.d_q + (size_t)slot * sa.s.dim_pad);
+                for (uint32_t j = lane; j < nv4; j += 64)
+                    dst[j] = src[j];
+                if (lane == 0) {
+                    sa.d_K[slot] = sa.h_kl[2 * slot];
+                    sa.d_L[slot] = sa.h_kl[2 * slot + 1];
+                }
+            }
+            __threadfence();
+            wave_sync();
+            if (lane < end - copied) {
+                const uint64_t i = copied + lane;
+                __hip_atomic_store(&sa.d_seq[(uint32_t)i & mask], i + 1, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            copied = end;
+            if (lane == 0)
+                __hip_atomic_store(&sa.d_ctl[2], copied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = (uint64_t)wall_clock64();
+        } else {
+            if ((uint64_t)wall_clock64() - last > sa.idle_ticks)
+                break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    if (lane == 0)
+        __hip_atomic_store(&sa.d_ctl[1], (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(512) void serve_kernel(ServeArgs sa)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    if (blockIdx.x == 0 && wave == 0) {
+        serve_dispatch(sa);
+        return;
+    }
+    const size_t per_wave = wave_lds_bytes(sa.s.dim_pad, sa.s.l_max, sa.s.hash_slots);
+    const WaveLds w = carve(smem + per_wave * wave, sa.s.dim_pad, sa.s.l_max);
+    const uint32_t mask = sa.slots - 1u;
+    while (true) {
+        uint64_t idx = 0;
+        if (lane == 0)
+            idx = atomicAdd(reinterpret_cast<unsigned long long *>(&sa.d_ctl[0]), 1ull);
+        idx = uniform64(idx);
+        const uint32_t slot = (uint32_t)idx & mask;
+        // Wait until the dispatcher has copied request idx into its slot.  A
+        // worker also leaves when the dispatcher's heartbeat has been silent
+        // for idle_ticks (the dispatcher is gone), so every wave exits.
+        uint32_t ready = 0, stop = 0, nap = 1;
+        const uint64_t wait_from = (uint64_t)wall_clock64();
+        while (true) {
+            if (lane == 0) {
+                ready = __hip_atomic_load(&sa.d_seq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == idx + 1;
+                stop = !ready && __hip_atomic_load(&sa.d_ctl[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (!ready && !stop) {
+                    uint64_t hb = __hip_atomic_load(&sa.d_ctl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    hb = hb > wait_from ? hb : wait_from;
+                    stop = (uint64_t)wall_clock64() - hb > sa.idle_ticks;
+                }
+            }
+            ready = uniform(ready);
+            stop = uniform(stop);
+            if (ready || stop)
+                break;
+            __builtin_amdgcn_s_sleep(8);
+            nap = nap < 8 ? nap + 1 : nap;
+            for (uint32_t z = 1; z < nap; z++)
+                __builtin_amdgcn_s_sleep(8);
+        }
+        if (!ready)
+            break;
+        search_one<NBLK, TAILI, METRIC>(sa.s, w, slot);
+        // Every lane's result stores reach the host before the done flag.
+        __threadfence_system();
+        wave_sync();
+        if (lane == 0)
+            __hip_atomic_store(&sa.h_done[slot], idx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Brute force: a block holds kBfQ queries in LDS and streams the base in
@@ -716,15 +1082,50 @@ hipError_t dispatch_dim(uint32_t dim_pad, Args &&...args)
 
 template <int NBLK, int TAILI, int METRIC>
 struct SearchLaunch {
+    template <int EW>
+    static hipError_t go(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
+    {
+        const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, EW) * wpb;
+        hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC, EW>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+        return hipGetLastError();
+    }
     static hipError_t run(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
-        const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots) * wpb;
-        hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+        switch (a.expand_width) {
+        case 2:
+            return go<2>(a, blocks, wpb, st);
+        case 4:
+            return go<4>(a, blocks, wpb, st);
+        default:
+            return go<1>(a, blocks, wpb, st);
+        }
+    }
+    template <int EW>
+    static hipError_t attr1(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC, EW>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
+    static hipError_t attr(size_t max_lds)
+    {
+        hipError_t e;
+        if ((e = attr1<1>(max_lds)) != hipSuccess || (e = attr1<2>(max_lds)) != hipSuccess)
+            return e;
+        return attr1<4>(max_lds);
+    }
+};
+
+template <int NBLK, int TAILI, int METRIC>
+struct ServeLaunch {
+    static hipError_t run(const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
+    {
+        const size_t lds = wave_lds_bytes(a.s.dim_pad, a.s.l_max, a.s.hash_slots) * wpb;
+        hipLaunchKernelGGL((serve_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
         return hipGetLastError();
     }
     static hipError_t attr(size_t max_lds)
     {
-        return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC>),
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(serve_kernel<NBLK, TAILI, METRIC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
     }
 };
@@ -760,9 +1161,9 @@ hipError_t attr_all(size_t max_lds)
 
 }  // namespace
 
-size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots)
+size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots, uint32_t expand_width)
 {
-    return wave_lds_bytes(dim_pad, l_max, hash_slots);
+    return wave_lds_bytes(dim_pad, l_max, hash_slots, expand_width);
 }
 
 size_t bruteforce_lds_bytes(uint32_t dim_pad) { return bf_lds_bytes(dim_pad); }
@@ -775,6 +1176,13 @@ hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint3
     if (metric == SS_METRIC_INNER_PRODUCT)
         return dispatch_dim<SearchLaunch, SS_METRIC_INNER_PRODUCT>(a.dim_pad, a, blocks, wpb, stream);
     return dispatch_dim<SearchLaunch, SS_METRIC_L2>(a.dim_pad, a, blocks, wpb, stream);
+}
+
+hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t stream)
+{
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<ServeLaunch, SS_METRIC_INNER_PRODUCT>(a.s.dim_pad, a, blocks, wpb, stream);
+    return dispatch_dim<ServeLaunch, SS_METRIC_L2>(a.s.dim_pad, a, blocks, wpb, stream);
 }
 
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream)
@@ -803,6 +1211,10 @@ hipError_t init_kernel_attributes(size_t max_lds)
     if ((e = attr_all<SearchLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<SearchLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<ServeLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<ServeLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<BfLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;

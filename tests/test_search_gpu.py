@@ -209,3 +209,43 @@ def test_invalid_arguments_rejected(small):
     bad[3, 0] = small["base"].shape[0] + 5
     with pytest.raises(SSError):
         ix.set_graph(bad, small["ep"])
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("seed", range(4))
+def test_wide_expansion_matches_model(width, seed):
+    """Expanding the first 2 or 4 unexplored entries per step: the device
+    result equals the step model of tests/merge_model.py exactly."""
+    import oracle
+    from merge_model import merge_search
+    rng = np.random.default_rng(50 + seed)
+    num = 300
+    base = rng.integers(-3, 4, (num, 16)).astype(np.float32) if seed % 2 else \
+        rng.standard_normal((num, 16)).astype(np.float32)
+    adj = random_graph(rng, num, 12)
+    ep = int(rng.integers(0, num))
+    q = rng.standard_normal((12, 16)).astype(np.float32)
+    ix = _index(base, adj, ep)
+    for L in (8, 40):
+        ids, d, st = ix.search(q, 8, L, visited_slots=4096, expand_width=width)
+        for i in range(q.shape[0]):
+            m = merge_search(base, adj, ep, q[i], 8, L, lambda x, y: np.float32(oracle.dist(0, x, y)),
+                             width=width)
+            assert np.array_equal(ids[i], m[0]), (seed, L, i)
+            assert np.array_equal(d[i].view(np.uint32), m[1].view(np.uint32))
+            assert st["ndc"][i] == m[2] and st["hops"][i] == m[3]
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_wide_expansion_recall(c0, width):
+    import oracle
+    from scattersearch_amd import metrics
+    gt, _ = oracle.bruteforce(c0["base"], c0["queries"], 10)
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    exact = ix.search(c0["queries"], 10, 64)
+    wide = ix.search(c0["queries"], 10, 64, expand_width=width)
+    r1, rw = metrics.recall(exact[0], gt, 10), metrics.recall(wide[0], gt, 10)
+    assert rw >= r1 - 0.02
+    assert wide[2]["hops"].mean() >= exact[2]["hops"].mean() * 0.8  # same work, fewer steps
+    for r in range(wide[0].shape[0]):
+        assert (np.diff(wide[1][r]) >= 0).all() and len(set(wide[0][r].tolist())) == 10

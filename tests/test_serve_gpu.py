@@ -1,0 +1,88 @@
+"""Continuous-batching serving on the resident kernel: every result equals
+the same query's result alone (the batch search and the CPU oracle), in any
+mix of arrivals, K and L."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(d):
+    from scattersearch_amd import Index
+    ix = Index(d["base"])
+    ix.set_graph(d["adj"], d["ep"])
+    return ix
+
+
+def test_replay_results_equal_alone(small):
+    import oracle
+    from scattersearch_amd import serve
+    ix = _index(small)
+    rng = np.random.default_rng(5)
+    q = np.concatenate([small["queries"]] * 4)  # repeats mix with other requests
+    nq = q.shape[0]
+    K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(max(k, 16), 129) for k in K], dtype=np.uint32)
+    arr = serve.poisson_arrivals(nq, 20000, seed=1)
+    ids, ds, lat, st = serve.replay(ix, q, K, L, arr, k_max=32, l_max=128, idle_timeout_ms=1000,
+                                    visited_slots=8192)
+    assert (lat > 0).all()
+    for i in range(nq):
+        k = int(K[i])
+        o_ids, o_d, o_ndc, o_hops = oracle.search(small["base"], small["adj"], small["ep"], q[i], k, int(L[i]))
+        assert np.array_equal(ids[i, :k], o_ids), f"request {i} K={k} L={L[i]}"
+        assert np.array_equal(ds[i, :k].view(np.uint32), o_d.view(np.uint32))
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+        assert st["hops"][i] == o_hops and st["ndc"][i] == o_ndc
+
+
+def test_submit_poll_and_idle_restart(small):
+    from scattersearch_amd import serve
+    ix = _index(small)
+    alone, _, _ = ix.search(small["queries"][:8], 10, 48, visited_slots=8192)
+    with serve.Server(ix, 10, 48, idle_timeout_ms=50, visited_slots=8192) as srv:
+        got = {}
+        for rnd in range(2):
+            tickets = [srv.submit(small["queries"][i], 10, 48) for i in range(8)]
+            t0 = time.time()
+            while len(got) < 8 * (rnd + 1) and time.time() - t0 < 20:
+                tk, ids, ds, st = srv.poll()
+                for t, r in zip(tk, ids):
+                    got[int(t)] = r.copy()
+            assert len(got) == 8 * (rnd + 1)
+            for j, t in enumerate(tickets):
+                assert np.array_equal(got[t], alone[j])
+            time.sleep(0.3)  # the resident kernel idles out; the next submit restarts it
+
+
+def test_admission_limits(small):
+    from scattersearch_amd import SSError, serve
+    from scattersearch_amd._lib import SS_EBUSY, SS_EINVAL
+    ix = _index(small)
+    with serve.Server(ix, 10, 48, ring_slots=64, idle_timeout_ms=500) as srv:
+        with pytest.raises(SSError) as e:
+            srv.submit(small["queries"][0], 20, 48)  # K above k_max
+        assert e.value.code == SS_EINVAL
+        with pytest.raises(SSError):
+            srv.submit(small["queries"][0], 10, 64)  # L above l_max
+        for i in range(64):
+            srv.submit(small["queries"][i % 64], 10, 48)
+        with pytest.raises(SSError) as e:
+            srv.submit(small["queries"][0], 10, 48)  # 64 unpolled requests fill the ring
+        assert e.value.code == SS_EBUSY
+        n, t0 = 0, time.time()
+        while n < 64 and time.time() - t0 < 20:
+            n += len(srv.poll()[0])
+        assert n == 64
+        srv.submit(small["queries"][0], 10, 48)  # room again
+
+
+def test_stop_with_requests_in_flight(small):
+    from scattersearch_amd import serve
+    ix = _index(small)
+    srv = serve.Server(ix, 10, 48, idle_timeout_ms=500)
+    for i in range(32):
+        srv.submit(small["queries"][i], 10, 48)
+    srv.stop()
