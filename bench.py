@@ -163,7 +163,28 @@ def main():
     if chosen is None:
         chosen = L_list[-1]
         log(f"no L reached recall {args.recall_target}; reporting the largest L")
+    elif not args.L:
+        # refine between the last L below the target and the first above it
+        lo = max([r["L"] for r in sweep if r["L"] < chosen] or [K])
+        for Lr in range(((lo + 8) // 8) * 8, chosen, 8):
+            wall, kern_ms = timed(Lr, 3, 1)
+            ids = d_ids.cpu().numpy().view(np.uint32)
+            rec = metrics.recall(ids, gt, K)
+            st = stats_host()
+            row = dict(L=Lr, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
+                       mean_ndc=round(float(st["ndc"].mean()), 1), mean_hops=round(float(st["hops"].mean()), 1),
+                       lossy=int(st["lossy"].sum()))
+            sweep.append(row)
+            log(f"refine {row}")
+            if rec >= args.recall_target:
+                chosen = Lr
+                break
     L = chosen
+    sweep.sort(key=lambda r: r["L"])
+
+    # Autotune launch parameters for this L on a sample (exact results kept).
+    tuned = ix.autotune(queries[:min(nq, 2000)], K, L, max_recall_loss=0.0)
+    log(f"autotuned for L={L}: {tuned}")
 
     # Timed region: K steps of one batched launch over all queries.
     wall, kern_ms = timed(L, args.steps, args.warmup)
@@ -200,8 +221,12 @@ def main():
 
     serving = None
     if args.serve_rates:
-        serving = serve_sweep(ix, queries, gt, w, [float(r) for r in args.serve_rates.split(",")],
-                              args.serve_requests)
+        try:
+            serving = serve_sweep(ix, queries, gt, w, [float(r) for r in args.serve_rates.split(",")],
+                                  args.serve_requests)
+        except Exception as exc:  # the batch numbers above stay valid
+            serving = {"error": str(exc)}
+            log(f"serving sweep failed: {exc}")
 
     cpu = None
     if not args.no_cpu:
@@ -222,7 +247,8 @@ def main():
         "data": f"synthetic: seeded {w.centres}-component Gaussian mixture (seed {w.seed}), graph built on the GPU",
         "config": {"workload": w.name, "num": num, "dim": w.dim, "nq": nq, "R": w.R, "K": K, "L": L,
                    "metric": "L2" if w.metric == 0 else "inner_product", "recall@10": round(rec, 4),
-                   "mean_ndc": round(float(st["ndc"].mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1)},
+                   "mean_ndc": round(float(st["ndc"].mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1),
+                   "launch": {k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots")}},
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
         "serving": serving,
@@ -243,8 +269,8 @@ def main():
 
 def serve_sweep(ix, queries, gt10, w, rates, nreq):
     """Online serving (the C4 workload): Poisson arrivals at each rate into
-    the resident kernel, K = 10 (80%) or 100 (20%), L uniform in 64..512;
-    latency = completion seen by the host minus arrival."""
+    the continuous-batching loop, K = 10 (80%) or 100 (20%), L uniform in
+    64..512; latency = completion seen by the host minus arrival."""
     from scattersearch_amd import metrics, serve
     rng = np.random.default_rng(20261207)
     qi = np.arange(nreq) % queries.shape[0]
@@ -255,7 +281,7 @@ def serve_sweep(ix, queries, gt10, w, rates, nreq):
     for r in rates:
         arr = serve.poisson_arrivals(nreq, r, seed=int(r))
         t = time.perf_counter()
-        ids, ds, lat, st = serve.replay(ix, Q, K, L, arr, k_max=100, l_max=512, idle_timeout_ms=2000)
+        ids, ds, lat, st = serve.replay(ix, Q, K, L, arr, k_max=100, l_max=512)
         wall = time.perf_counter() - t
         span_s = (arr[-1] + lat[-1]) * 1e-6
         rec = metrics.recall(ids[:, :10], gt10[qi], 10)
@@ -266,8 +292,8 @@ def serve_sweep(ix, queries, gt10, w, rates, nreq):
                "mean_ndc": round(float(st["ndc"].mean()), 1), "wall_s": round(wall, 2)}
         out.append(row)
         log(f"serve {row}")
-    return {"workload": "resident kernel, Poisson arrivals, K=10 (80%) / 100 (20%), L uniform 64..512",
-            "rates": out}
+    return {"workload": "continuous batching into batched launches on 4 streams, Poisson arrivals, "
+                        "K=10 (80%) / 100 (20%), L uniform 64..512", "rates": out}
 
 
 def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):

@@ -160,12 +160,24 @@ int ss_bruteforce_device(ss_index *index, const float *d_queries, uint32_t nq, u
  * written to host arrays [num][k]. */
 int ss_knn_graph(ss_index *index, uint32_t k, uint32_t *out_ids, float *out_dists);
 
+/* Autotuner: times launch configurations (expansion width, waves per block,
+ * visited-hash size) for pool size L on a sample of queries and keeps the
+ * fastest whose recall@K against exact brute force on the sample is within
+ * max_recall_loss of the exact search (max_recall_loss = 0 keeps exact
+ * results: expansion width 1).  The choice is stored in the index for L's
+ * power-of-two bucket and used by later searches that leave those fields 0. */
+int ss_autotune(ss_index *index, const float *queries, uint32_t nq, uint32_t K, uint32_t L,
+                float max_recall_loss, ss_search_params *best, float *best_ms,
+                float *best_recall);
+
 /* ---------------------------------------------------------------------- */
 /* Continuous-batching serving (tests/online_parallel_search.cpp:78-199,    */
-/* tests/online_pipeline_search.cpp:136-172).  A kernel stays resident on   */
-/* the GPU with the graph and per-query state; requests submitted while     */
-/* others run are admitted as wavefronts free up and overlap on the device. */
-/* One thread may submit while another polls.                               */
+/* tests/online_pipeline_search.cpp:136-172).  Requests go into a ring in   */
+/* pinned host memory; a dispatcher thread admits everything that has      */
+/* arrived into the next batched launch as soon as one of `streams` launch  */
+/* slots frees up, so requests arriving while others run overlap with them  */
+/* on the GPU; graph and vectors stay resident.  One thread may submit      */
+/* while another polls.                                                     */
 /* ---------------------------------------------------------------------- */
 typedef struct ss_server ss_server;
 
@@ -173,13 +185,11 @@ typedef struct ss_serve_config {
     uint32_t k_max;           /* largest K a request may ask for */
     uint32_t l_max;           /* largest L (sizes each wavefront's LDS pool) */
     uint32_t ring_slots;      /* most requests in flight (power of two); 0 = 16384 */
-    uint32_t visited_slots;   /* per-wavefront visited hash; 0 = tuned */
+    uint32_t visited_slots;   /* per-query visited hash; 0 = tuned */
     uint32_t waves_per_block; /* 0 = tuned */
-    uint32_t blocks;          /* resident grid; 0 = fill the GPU */
-    uint32_t idle_timeout_ms; /* the resident kernel exits after this long
-                                 without requests and restarts on the next
-                                 one; 0 = 2000 */
-    uint32_t reserved;
+    uint32_t blocks;          /* grid cap per launch; 0 = fill the GPU */
+    uint32_t streams;         /* launches in flight at once, 1..8; 0 = 4 */
+    uint32_t max_batch;       /* most requests per launch; 0 = 2048 */
 } ss_serve_config;
 
 int ss_serve_start(ss_index *index, const ss_serve_config *config, ss_server **out);

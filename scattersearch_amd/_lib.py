@@ -57,8 +57,8 @@ class ServeConfig(ctypes.Structure):
         ("visited_slots", ctypes.c_uint32),
         ("waves_per_block", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
-        ("idle_timeout_ms", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("streams", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32),
     ]
 
 
@@ -115,6 +115,8 @@ _SIGS = {
     "ss_bruteforce": (cint, [vp, f32p, u32, u32, u32p, u32p, f32p]),
     "ss_bruteforce_device": (cint, [vp, vp, u32, u32, cint, u32, vp, vp, vp]),
     "ss_knn_graph": (cint, [vp, u32, u32p, f32p]),
+    "ss_autotune": (cint, [vp, f32p, u32, u32, u32, ctypes.c_float, ctypes.POINTER(SearchParams),
+                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "ss_serve_start": (cint, [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]),
     "ss_submit": (cint, [vp, f32p, u32, u32, ctypes.POINTER(ctypes.c_uint64)]),
     "ss_poll": (cint, [vp, u32, ctypes.POINTER(ctypes.c_uint64), u32p, f32p,

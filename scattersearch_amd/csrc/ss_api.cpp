@@ -59,6 +59,14 @@ int hip_fail(hipError_t e, const char *what)
 
 }  // namespace
 
+static uint32_t next_pow2(uint32_t x)
+{
+    uint32_t p = 1;
+    while (p < x)
+        p <<= 1;
+    return p;
+}
+
 struct ss_index {
     uint32_t dim = 0, dim_pad = 0, num = 0;
     int metric = SS_METRIC_L2;
@@ -78,6 +86,12 @@ struct ss_index {
     DevBuf<float> dists;
     DevBuf<ss_query_stats> stats;
     DevBuf<uint32_t> path;
+    // Launch parameters chosen by ss_autotune, per pool-size bucket
+    // (next power of two of L); dim and degree are fixed per index.
+    struct Tuned {
+        uint32_t l_bucket, expand_width, waves_per_block, visited_slots;
+    };
+    std::vector<Tuned> tuned;
     // hipGraph cache of the last batched launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -404,12 +418,27 @@ struct Shape {
     size_t lds_wave;
 };
 
-static uint32_t next_pow2(uint32_t x)
+
+// Fields left at 0 take the autotuned value for this pool size, if any.
+static ss_search_params resolve_params(const ss_index *ix, uint32_t l_max, const ss_search_params *p)
 {
-    uint32_t p = 1;
-    while (p < x)
-        p <<= 1;
-    return p;
+    ss_search_params r{};
+    if (p)
+        r = *p;
+    const uint32_t bucket = next_pow2(l_max);
+    for (const auto &t : ix->tuned) {
+        if (t.l_bucket != bucket)
+            continue;
+        if (!r.expand_width)
+            r.expand_width = t.expand_width;
+        if (!r.waves_per_block)
+            r.waves_per_block = t.waves_per_block;
+        if (!r.visited_slots)
+            r.visited_slots = t.visited_slots;
+    }
+    if (!r.expand_width)
+        r.expand_width = 1;
+    return r;
 }
 
 static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const ss_search_params *p, Shape *s)
@@ -480,6 +509,10 @@ int index_view(const ss_index *ix, IndexView *v)
     return SS_OK;
 }
 
+int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
+                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
+                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue);
+
 int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,
               uint32_t blocks, uint32_t *wpb, uint32_t *slots, uint32_t *blocks_out)
 {
@@ -511,7 +544,7 @@ static uint32_t log2u(uint32_t x)
 static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
                       uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
                       uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st,
-                      uint32_t *d_path = nullptr, uint32_t path_cap = 0)
+                      uint32_t *d_path = nullptr, uint32_t path_cap = 0, uint32_t *d_queue = nullptr)
 {
     if (!ix->has_graph) {
         set_error("search: no graph loaded");
@@ -532,6 +565,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         set_error("search: unknown mode");
         return SS_EINVAL;
     }
+    const ss_search_params resolved = resolve_params(ix, l_max, p);
+    p = &resolved;
     Shape sh;
     int rc = choose_shape(ix, l_max, mode == SS_MODE_PER_QUERY ? 1 : nq, p, &sh);
     if (rc)
@@ -571,8 +606,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         }
         return SS_OK;
     }
-    a.queue = ix->d_queue;
-    const bool use_graph = p && p->use_graph && !d_path;
+    a.queue = d_queue ? d_queue : ix->d_queue;
+    const bool use_graph = p && p->use_graph && !d_path && !d_queue;
     if (use_graph) {
         const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
                           ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
@@ -612,7 +647,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             return hip_fail(e, "hipGraphLaunch");
         return SS_OK;
     }
-    if ((e = ss::launch_queue_reset(ix->d_queue, st)) != hipSuccess)
+    if ((e = ss::launch_queue_reset(a.queue, st)) != hipSuccess)
         return hip_fail(e, "queue reset");
     if ((e = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st)) != hipSuccess)
         return hip_fail(e, "search launch");
@@ -847,4 +882,154 @@ int ss_knn_graph(ss_index *ix, uint32_t k, uint32_t *out_ids, float *out_dists)
     return bruteforce_host(ix, ix->d_base, ix->num, k, 1, out_ids, out_dists);
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner: times candidate launch configurations on a sample of queries and
+// keeps the fastest whose recall@K (against exact brute force on the sample)
+// is within max_recall_loss of the exact search's.  Candidates: expansion
+// width (1, and 2 and 4 when a recall loss is allowed), waves per block and
+// visited-hash size.  The choice is stored in the index for L's bucket and
+// used by every later search that leaves those parameters at 0.
+// ---------------------------------------------------------------------------
+int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uint32_t L, float max_recall_loss,
+                ss_search_params *best_out, float *best_ms_out, float *best_recall_out)
+{
+    if (!ix || !queries || nq == 0) {
+        set_error("ss_autotune: bad argument");
+        return SS_EINVAL;
+    }
+    if (!ix->has_graph) {
+        set_error("ss_autotune: no graph loaded");
+        return SS_ESTATE;
+    }
+    if (K == 0 || K > L || L > SS_MAX_L || K > ss::bruteforce_kmax()) {
+        set_error("ss_autotune: need 1 <= K <= L <= 512 and K <= 128");
+        return SS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    hipError_t e;
+    const size_t nk = (size_t)nq * K;
+    DevBuf<float> dq, dd, gd;
+    DevBuf<uint32_t> di, gi;
+    if ((e = dq.reserve((size_t)nq * ix->dim_pad)) || (e = dd.reserve(nk)) || (e = di.reserve(nk)) ||
+        (e = gd.reserve(nk)) || (e = gi.reserve(nk)))
+        return hip_fail(e, "ss_autotune: allocation");
+    std::vector<float> padded((size_t)nq * ix->dim_pad, 0.0f);
+    for (uint32_t i = 0; i < nq; i++)
+        std::memcpy(&padded[(size_t)i * ix->dim_pad], queries + (size_t)i * ix->dim, ix->dim * sizeof(float));
+    if ((e = hipMemcpy(dq.ptr, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "ss_autotune: upload");
+    int rc = ss_bruteforce_device(ix, dq.ptr, nq, K, 0, 0, gi.ptr, gd.ptr, ix->stream);
+    if (rc)
+        return rc;
+    std::vector<uint32_t> gt(nk), got(nk);
+    if ((e = hipMemcpyAsync(gt.data(), gi.ptr, nk * 4, hipMemcpyDeviceToHost, ix->stream)) ||
+        (e = hipStreamSynchronize(ix->stream)))
+        return hip_fail(e, "ss_autotune: ground truth");
+    auto recall = [&](const std::vector<uint32_t> &r) {
+        size_t hit = 0;
+        for (uint32_t q = 0; q < nq; q++)
+            for (uint32_t j = 0; j < K; j++)
+                for (uint32_t g = 0; g < K; g++)
+                    if (r[(size_t)q * K + j] == gt[(size_t)q * K + g]) {
+                        hit++;
+                        break;
+                    }
+        return (float)hit / (float)nk;
+    };
+    hipEvent_t ev0, ev1;
+    if ((e = hipEventCreate(&ev0)) || (e = hipEventCreate(&ev1)))
+        return hip_fail(e, "ss_autotune: events");
+    struct Cand {
+        ss_search_params p;
+        float ms, rec;
+    };
+    std::vector<Cand> done;
+    const uint32_t widths[3] = {1, 2, 4};
+    const uint32_t wpbs[4] = {1, 2, 4, 8};
+    const uint32_t slotss[4] = {1024, 2048, 4096, 8192};
+    float exact_recall = -1.0f;
+    for (uint32_t ew : widths) {
+        if (ew > 1 && !(max_recall_loss > 0.0f))
+            continue;
+        for (uint32_t slots : slotss) {
+            for (uint32_t wpb : wpbs) {
+                ss_search_params p{};
+                p.expand_width = ew;
+                p.waves_per_block = wpb;
+                p.visited_slots = slots;
+                Shape sh;
+                if (choose_shape(ix, L, nq, &p, &sh) != SS_OK)
+                    continue;
+                if ((rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr,
+                                     ix->stream)))
+                    break;
+                float total = 0.0f;
+                for (int rep = 0; rep < 3 && !rc; rep++) {
+                    float ms = 0.0f;
+                    if ((e = hipEventRecord(ev0, ix->stream)) ||
+                        (rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr,
+                                         ix->stream)) ||
+                        (e = hipEventRecord(ev1, ix->stream)) || (e = hipEventSynchronize(ev1)) ||
+                        (e = hipEventElapsedTime(&ms, ev0, ev1))) {
+                        if (!rc)
+                            rc = hip_fail(e, "ss_autotune: timing");
+                        break;
+                    }
+                    total += ms;
+                }
+                if (rc)
+                    break;
+                if ((e = hipMemcpy(got.data(), di.ptr, nk * 4, hipMemcpyDeviceToHost)) != hipSuccess) {
+                    rc = hip_fail(e, "ss_autotune: results");
+                    break;
+                }
+                const float rec = recall(got);
+                if (ew == 1)
+                    exact_recall = rec;
+                done.push_back(Cand{p, total / 3.0f, rec});
+            }
+            if (rc)
+                break;
+        }
+        if (rc)
+            break;
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (rc)
+        return rc;
+    const Cand *best = nullptr;
+    for (const Cand &c : done)
+        if (c.rec >= exact_recall - max_recall_loss && (!best || c.ms < best->ms))
+            best = &c;
+    if (!best) {
+        set_error("ss_autotune: no configuration fits this L");
+        return SS_EINVAL;
+    }
+    const uint32_t bucket = next_pow2(L);
+    ix->tuned.erase(std::remove_if(ix->tuned.begin(), ix->tuned.end(),
+                                   [&](const ss_index::Tuned &t) { return t.l_bucket == bucket; }),
+                    ix->tuned.end());
+    ix->tuned.push_back(ss_index::Tuned{bucket, best->p.expand_width, best->p.waves_per_block, best->p.visited_slots});
+    if (best_out)
+        *best_out = best->p;
+    if (best_ms_out)
+        *best_ms_out = best->ms;
+    if (best_recall_out)
+        *best_recall_out = best->rec;
+    return SS_OK;
+}
+
 }  // extern "C"
+
+namespace ss {
+// A batched search on caller-owned device buffers with its own work-queue
+// counter, so that several searches can run at once on different streams.
+int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
+                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
+                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue)
+{
+    return run_search(ix, d_queries, nq, d_K, d_L, k_max, l_max, k_stride, p, d_ids, d_dists, d_stats, st, nullptr, 0,
+                      d_queue);
+}
+}  // namespace ss

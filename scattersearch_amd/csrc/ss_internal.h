@@ -40,27 +40,6 @@ struct SearchArgs {
     uint32_t expand_width; // 1 (exact), 2 or 4 pool entries expanded per step
 };
 
-// Resident serving kernel: one dispatcher wavefront copies published
-// requests from the host ring (fine-grained pinned memory) into device
-// slots; every other wavefront claims request indices from a device counter,
-// waits for its slot, searches it and writes the result and a done flag to
-// the host.
-struct ServeArgs {
-    SearchArgs s;            // queries = d_q, Ks = d_K, Ls = d_L, outputs = host result ring
-    const float *h_q;        // [slots][dim_pad] request vectors (host)
-    const uint32_t *h_kl;    // [slots][2] K, L (host)
-    const uint64_t *h_head;  // requests published by the host
-    const uint32_t *h_stop;  // host asks the kernel to exit
-    uint64_t *h_done;        // [slots] request index + 1 once its result is written
-    float *d_q;              // [slots][dim_pad]
-    uint32_t *d_K;           // [slots]
-    uint32_t *d_L;           // [slots]
-    uint64_t *d_seq;         // [slots] request index + 1 once copied to the device
-    uint64_t *d_ctl;         // [0] claim counter [1] stop [2] requests copied [3] dispatcher heartbeat
-    uint32_t slots;          // power of two
-    uint64_t idle_ticks;     // wall-clock ticks without requests before the kernel exits
-};
-
 struct BruteArgs {
     const float *base;
     const float *queries;
@@ -82,8 +61,6 @@ size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
                          uint32_t waves_per_block, hipStream_t stream);
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream);
-hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t waves_per_block,
-                        hipStream_t stream);
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream);
 hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t dim,
                            uint32_t dim_pad, hipStream_t stream);

@@ -749,127 +749,6 @@ __global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
 __global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
 
 // ---------------------------------------------------------------------------
-// Resident serving kernel.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t uniform64(uint64_t v)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Copies published requests into device slots and publishes them to the
-// workers; exits on the host's stop flag or after idle_ticks without a new
-// request, and then raises the device stop flag every worker watches.
-__device__ void serve_dispatch(const ServeArgs &sa)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t mask = sa.slots - 1u;
-    const uint32_t nv4 = sa.s.dim_pad / 4;
-    uint64_t copied = uniform64(__hip_atomic_load(&sa.d_ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    uint64_t last = (uint64_t)wall_clock64();
-    while (true) {
-        uint32_t stop = 0;
-        uint64_t head = 0;
-        if (lane == 0) {
-            stop = __hip_atomic_load(sa.h_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            head = __hip_atomic_load(sa.h_head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        stop = uniform(stop);
-        head = uniform64(head);
-        if (lane == 0)
-            __hip_atomic_store(&sa.d_ctl[3], (uint64_t)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (stop)
-            break;
-        if (head > copied) {
-            const uint64_t end = head - copied > 64 ? copied + 64 : head;
-            for (uint64_t i = copied; i < end; i++) {
-                const uint32_t slot = (uint32_t)i & mask;
-                const float4 *src = reinterpret_cast<const float4 *>(sa.h_q + (size_t)slot * sa.s.dim_pad);
-                float4 *dst = reinterpret_cast<float4 *>(sa.d_q + (size_t)slot * sa.s.dim_pad);
-                for (uint32_t j = lane; j < nv4; j += 64)
-                    dst[j] = src[j];
-                if (lane == 0) {
-                    sa.d_K[slot] = sa.h_kl[2 * slot];
-                    sa.d_L[slot] = sa.h_kl[2 * slot + 1];
-                }
-            }
-            __threadfence();
-            wave_sync();
-            if (lane < end - copied) {
-                const uint64_t i = copied + lane;
-                __hip_atomic_store(&sa.d_seq[(uint32_t)i & mask], i + 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-            copied = end;
-            if (lane == 0)
-                __hip_atomic_store(&sa.d_ctl[2], copied, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            last = (uint64_t)wall_clock64();
-        } else {
-            if ((uint64_t)wall_clock64() - last > sa.idle_ticks)
-                break;
-            __builtin_amdgcn_s_sleep(2);
-        }
-    }
-    if (lane == 0)
-        __hip_atomic_store(&sa.d_ctl[1], (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int NBLK, int TAILI, int METRIC>
-__global__ __launch_bounds__(512) void serve_kernel(ServeArgs sa)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
-    if (blockIdx.x == 0 && wave == 0) {
-        serve_dispatch(sa);
-        return;
-    }
-    const size_t per_wave = wave_lds_bytes(sa.s.dim_pad, sa.s.l_max, sa.s.hash_slots);
-    const WaveLds w = carve(smem + per_wave * wave, sa.s.dim_pad, sa.s.l_max);
-    const uint32_t mask = sa.slots - 1u;
-    while (true) {
-        uint64_t idx = 0;
-        if (lane == 0)
-            idx = atomicAdd(reinterpret_cast<unsigned long long *>(&sa.d_ctl[0]), 1ull);
-        idx = uniform64(idx);
-        const uint32_t slot = (uint32_t)idx & mask;
-        // Wait until the dispatcher has copied request idx into its slot.  A
-        // worker also leaves when the dispatcher's heartbeat has been silent
-        // for idle_ticks (the dispatcher is gone), so every wave exits.
-        uint32_t ready = 0, stop = 0, nap = 1;
-        const uint64_t wait_from = (uint64_t)wall_clock64();
-        while (true) {
-            if (lane == 0) {
-                ready = __hip_atomic_load(&sa.d_seq[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == idx + 1;
-                stop = !ready && __hip_atomic_load(&sa.d_ctl[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0;
-                if (!ready && !stop) {
-                    uint64_t hb = __hip_atomic_load(&sa.d_ctl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    hb = hb > wait_from ? hb : wait_from;
-                    stop = (uint64_t)wall_clock64() - hb > sa.idle_ticks;
-                }
-            }
-            ready = uniform(ready);
-            stop = uniform(stop);
-            if (ready || stop)
-                break;
-            __builtin_amdgcn_s_sleep(8);
-            nap = nap < 8 ? nap + 1 : nap;
-            for (uint32_t z = 1; z < nap; z++)
-                __builtin_amdgcn_s_sleep(8);
-        }
-        if (!ready)
-            break;
-        search_one<NBLK, TAILI, METRIC>(sa.s, w, slot);
-        // Every lane's result stores reach the host before the done flag.
-        __threadfence_system();
-        wave_sync();
-        if (lane == 0)
-            __hip_atomic_store(&sa.h_done[slot], idx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Brute force: a block holds kBfQ queries in LDS and streams the base in
 // tiles of 256 points, one point per lane.  Distances below a query's current
 // k-th (distance, id) go to a per-query candidate list; after each tile one
@@ -1116,21 +995,6 @@ struct SearchLaunch {
 };
 
 template <int NBLK, int TAILI, int METRIC>
-struct ServeLaunch {
-    static hipError_t run(const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
-    {
-        const size_t lds = wave_lds_bytes(a.s.dim_pad, a.s.l_max, a.s.hash_slots) * wpb;
-        hipLaunchKernelGGL((serve_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
-        return hipGetLastError();
-    }
-    static hipError_t attr(size_t max_lds)
-    {
-        return hipFuncSetAttribute(reinterpret_cast<const void *>(serve_kernel<NBLK, TAILI, METRIC>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-    }
-};
-
-template <int NBLK, int TAILI, int METRIC>
 struct BfLaunch {
     static hipError_t run(const BruteArgs &a, hipStream_t st)
     {
@@ -1178,13 +1042,6 @@ hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint3
     return dispatch_dim<SearchLaunch, SS_METRIC_L2>(a.dim_pad, a, blocks, wpb, stream);
 }
 
-hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t stream)
-{
-    if (metric == SS_METRIC_INNER_PRODUCT)
-        return dispatch_dim<ServeLaunch, SS_METRIC_INNER_PRODUCT>(a.s.dim_pad, a, blocks, wpb, stream);
-    return dispatch_dim<ServeLaunch, SS_METRIC_L2>(a.s.dim_pad, a, blocks, wpb, stream);
-}
-
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream)
 {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, queue);
@@ -1211,10 +1068,6 @@ hipError_t init_kernel_attributes(size_t max_lds)
     if ((e = attr_all<SearchLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<SearchLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
-        return e;
-    if ((e = attr_all<ServeLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
-        return e;
-    if ((e = attr_all<ServeLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<BfLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;

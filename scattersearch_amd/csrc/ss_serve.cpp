@@ -1,16 +1,25 @@
-// Continuous-batching serving loop over a resident GPU kernel.
+// Continuous-batching serving loop.
 //
 // The reference's online drivers push query ids into a mutex-guarded deque
 // and run each query with T CPU threads (tests/online_parallel_search.cpp:
 // 91-160) or fan it out over a ThreadPool so queries overlap
 // (tests/online_pipeline_search.cpp:136-172); latency is completion minus
-// receive time (online_parallel_search.cpp:154-158).  Here a request goes
-// into a ring in pinned host memory; a kernel that stays resident on the GPU
-// copies it to the device and hands it to the next free wavefront, so
-// requests that arrive while others run start at once on free slots and
-// overlap on the device.  Admission is by budget: every wavefront owns LDS
-// for a pool of l_max and results of k_max, so requests with larger K or L
-// are refused, and at most `ring_slots` requests are in flight.
+// receive time (online_parallel_search.cpp:154-158).
+//
+// Here submit() drops a request into a ring in pinned host memory.  A
+// dispatcher thread admits every request that has arrived into the next
+// batched search launch as soon as one of `streams` launch slots is free: it
+// copies the new requests' vectors, K and L to the device, runs the batched
+// search (each launch with its own work-queue counter) and copies the results
+// back, all asynchronously on that slot's stream.  So requests that arrive
+// while others are on the GPU start in the next launch and overlap with the
+// ones running, graph and vectors stay resident, and batches form by
+// themselves from whatever arrived during the previous launches.  Admission
+// is by budget: every launch's wavefronts are sized for k_max and l_max (LDS
+// pool and visited hash), so larger K or L are refused, and at most
+// `ring_slots` requests are in flight.  Every kernel launched here ends on its
+// own (no device-side waiting), so nothing the host does can leave the GPU
+// spinning.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -26,7 +35,6 @@ using ss::set_error;
 
 struct ss_index;
 namespace ss {
-// Accessors of the index (ss_api.cpp).
 struct IndexView {
     const float *d_base;
     const uint32_t *d_adj;
@@ -36,25 +44,21 @@ struct IndexView {
     bool has_graph;
 };
 int index_view(const ss_index *ix, IndexView *v);
-int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,
-              uint32_t blocks, uint32_t *wpb, uint32_t *slots, uint32_t *blocks_out);
+int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
+                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
+                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue);
 }  // namespace ss
 
 namespace {
 
-int hip_fail(hipError_t e, const char *what)
-{
-    set_error(std::string(what) + ": " + hipGetErrorString(e));
-    return e == hipErrorOutOfMemory ? SS_ENOMEM : SS_EDEVICE;
-}
+constexpr uint32_t kMaxStreams = 8;
 
-uint32_t log2u(uint32_t x)
-{
-    uint32_t r = 0;
-    while ((1u << r) < x)
-        r++;
-    return r;
-}
+std::string hip_msg(hipError_t e, const char *what) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+struct Launch {
+    bool busy = false;
+    uint64_t first = 0, count = 0;
+};
 
 }  // namespace
 
@@ -62,151 +66,171 @@ struct ss_server {
     ss_index *ix = nullptr;
     ss::IndexView v{};
     ss_serve_config cfg{};
-    uint32_t S = 0, mask = 0, wpb = 0, blocks = 0, hslots = 0;
-    int metric = 0;
-    // pinned, fine-grained host memory
+    uint32_t S = 0, mask = 0, ns = 0, max_batch = 0;
+    ss_search_params params{};
+    // pinned host rings
     float *h_q = nullptr;
-    uint32_t *h_kl = nullptr;
-    uint64_t *h_head = nullptr;
-    uint32_t *h_stop = nullptr;
-    uint64_t *h_done = nullptr;
+    uint32_t *h_K = nullptr, *h_L = nullptr;
     uint32_t *h_ids = nullptr;
     float *h_d = nullptr;
     ss_query_stats *h_stats = nullptr;
-    // device memory
+    std::vector<std::atomic<uint64_t>> done;  // request index + 1 once its result is on the host
+    // device rings
     float *d_q = nullptr;
     uint32_t *d_K = nullptr, *d_L = nullptr;
-    uint64_t *d_seq = nullptr, *d_ctl = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t exited = nullptr;
-    std::atomic<bool> launched{false};
-    uint32_t khz = 0;
-    // host bookkeeping: one producer (submit), one consumer (poll)
+    uint32_t *d_ids = nullptr;
+    float *d_d = nullptr;
+    ss_query_stats *d_stats = nullptr;
+    uint32_t *d_queues = nullptr;  // one work-queue counter per launch slot, 64 bytes apart
+    hipStream_t st[kMaxStreams] = {};
+    hipEvent_t ev[kMaxStreams] = {};
+    Launch slot[kMaxStreams];
+    std::thread dispatcher;
+    std::atomic<bool> stop{false};
+    std::atomic<int> error{SS_OK};
+    std::mutex err_mu;
+    std::string err_msg;
+    // producer / consumer counters
     std::atomic<uint64_t> submitted{0};
     std::atomic<uint64_t> completed{0};
-    uint64_t oldest = 0;
+    uint64_t dispatched = 0;  // dispatcher thread only
+    uint64_t oldest = 0;      // poller only
     std::vector<uint8_t> consumed;
-    std::mutex launch_mu;
+
+    explicit ss_server(uint32_t slots) : done(slots) {}
+
+    void fail(int code, const std::string &msg)
+    {
+        std::lock_guard<std::mutex> lk(err_mu);
+        if (error.load() == SS_OK) {
+            err_msg = msg;
+            error.store(code);
+        }
+    }
 };
 
 static void free_server(ss_server *s)
 {
     if (!s)
         return;
-    if (s->stream)
-        (void)hipStreamSynchronize(s->stream);
-    if (s->h_q)
-        (void)hipHostFree(s->h_q);
-    if (s->h_kl)
-        (void)hipHostFree(s->h_kl);
-    if (s->h_head)
-        (void)hipHostFree(s->h_head);
-    if (s->h_stop)
-        (void)hipHostFree(s->h_stop);
-    if (s->h_done)
-        (void)hipHostFree(s->h_done);
-    if (s->h_ids)
-        (void)hipHostFree(s->h_ids);
-    if (s->h_d)
-        (void)hipHostFree(s->h_d);
-    if (s->h_stats)
-        (void)hipHostFree(s->h_stats);
-    if (s->d_q)
-        (void)hipFree(s->d_q);
-    if (s->d_K)
-        (void)hipFree(s->d_K);
-    if (s->d_L)
-        (void)hipFree(s->d_L);
-    if (s->d_seq)
-        (void)hipFree(s->d_seq);
-    if (s->d_ctl)
-        (void)hipFree(s->d_ctl);
-    if (s->exited)
-        (void)hipEventDestroy(s->exited);
-    if (s->stream)
-        (void)hipStreamDestroy(s->stream);
+    for (uint32_t i = 0; i < kMaxStreams; i++) {
+        if (s->st[i])
+            (void)hipStreamSynchronize(s->st[i]);
+    }
+    auto hf = [](void *p) {
+        if (p)
+            (void)hipHostFree(p);
+    };
+    auto df = [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    };
+    hf(s->h_q);
+    hf(s->h_K);
+    hf(s->h_L);
+    hf(s->h_ids);
+    hf(s->h_d);
+    hf(s->h_stats);
+    df(s->d_q);
+    df(s->d_K);
+    df(s->d_L);
+    df(s->d_ids);
+    df(s->d_d);
+    df(s->d_stats);
+    df(s->d_queues);
+    for (uint32_t i = 0; i < kMaxStreams; i++) {
+        if (s->ev[i])
+            (void)hipEventDestroy(s->ev[i]);
+        if (s->st[i])
+            (void)hipStreamDestroy(s->st[i]);
+    }
     delete s;
 }
 
-template <typename T>
-static hipError_t host_alloc(T **p, size_t n)
+// Launch requests [first, first+count), which occupy contiguous ring slots.
+static int launch_batch(ss_server *s, uint32_t i, uint64_t first, uint32_t count)
 {
-    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(p), std::max<size_t>(n, 1) * sizeof(T),
-                                 hipHostMallocCoherent | hipHostMallocMapped);
-    if (e == hipSuccess)
-        std::memset(*p, 0, std::max<size_t>(n, 1) * sizeof(T));
-    return e;
-}
-
-// (Re)launch the resident kernel; the claim counter restarts at the number of
-// requests the previous kernel copied to the device.
-static int launch(ss_server *s)
-{
+    const uint32_t s0 = (uint32_t)first & s->mask;
+    const size_t dp = s->v.dim_pad, km = s->cfg.k_max;
+    hipStream_t st = s->st[i];
     hipError_t e;
-    uint64_t ctl[4] = {0, 0, 0, 0};
-    if ((e = hipMemcpy(ctl, s->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost)) != hipSuccess)
-        return hip_fail(e, "serve: read control block");
-    ctl[0] = ctl[2];
-    ctl[1] = 0;
-    ctl[3] = 0;
-    if ((e = hipMemcpy(s->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice)) != hipSuccess)
-        return hip_fail(e, "serve: reset control block");
-    __atomic_store_n(s->h_stop, 0u, __ATOMIC_RELEASE);
-    ss::ServeArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.s.base = s->v.d_base;
-    a.s.adj = s->v.d_adj;
-    a.s.queries = s->d_q;
-    a.s.Ks = s->d_K;
-    a.s.Ls = s->d_L;
-    a.s.k_default = s->cfg.k_max;
-    a.s.l_default = s->cfg.l_max;
-    a.s.nq = s->S;
-    a.s.dim_pad = s->v.dim_pad;
-    a.s.r_pad = s->v.r_pad;
-    a.s.ep = s->v.ep;
-    a.s.l_max = s->cfg.l_max;
-    a.s.hash_slots = s->hslots;
-    a.s.hash_shift = 32u - log2u(s->hslots);
-    a.s.k_stride = s->cfg.k_max;
-    a.s.out_ids = s->h_ids;
-    a.s.out_dists = s->h_d;
-    a.s.stats = s->h_stats;
-    a.h_q = s->h_q;
-    a.h_kl = s->h_kl;
-    a.h_head = s->h_head;
-    a.h_stop = s->h_stop;
-    a.h_done = s->h_done;
-    a.d_q = s->d_q;
-    a.d_K = s->d_K;
-    a.d_L = s->d_L;
-    a.d_seq = s->d_seq;
-    a.d_ctl = s->d_ctl;
-    a.slots = s->S;
-    a.idle_ticks = (uint64_t)s->cfg.idle_timeout_ms * s->khz;
-    if ((e = ss::launch_serve(s->metric, a, s->blocks, s->wpb, s->stream)) != hipSuccess)
-        return hip_fail(e, "serve kernel launch");
-    if ((e = hipEventRecord(s->exited, s->stream)) != hipSuccess)
-        return hip_fail(e, "hipEventRecord");
-    s->launched = true;
+    if ((e = hipMemcpyAsync(s->d_q + s0 * dp, s->h_q + s0 * dp, count * dp * sizeof(float), hipMemcpyHostToDevice,
+                            st)) ||
+        (e = hipMemcpyAsync(s->d_K + s0, s->h_K + s0, count * 4, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(s->d_L + s0, s->h_L + s0, count * 4, hipMemcpyHostToDevice, st))) {
+        s->fail(SS_EDEVICE, hip_msg(e, "serve: request upload"));
+        return SS_EDEVICE;
+    }
+    int rc = ss::search_launch(s->ix, s->d_q + s0 * dp, count, s->d_K + s0, s->d_L + s0, s->cfg.k_max, s->cfg.l_max,
+                               s->cfg.k_max, &s->params, s->d_ids + s0 * km, s->d_d + s0 * km, s->d_stats + s0, st,
+                               s->d_queues + 16 * i);
+    if (rc) {
+        s->fail(rc, "serve: search launch failed");
+        return rc;
+    }
+    if ((e = hipMemcpyAsync(s->h_ids + s0 * km, s->d_ids + s0 * km, count * km * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(s->h_d + s0 * km, s->d_d + s0 * km, count * km * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(s->h_stats + s0, s->d_stats + s0, count * sizeof(ss_query_stats), hipMemcpyDeviceToHost,
+                            st)) ||
+        (e = hipEventRecord(s->ev[i], st))) {
+        s->fail(SS_EDEVICE, hip_msg(e, "serve: result download"));
+        return SS_EDEVICE;
+    }
+    s->slot[i].busy = true;
+    s->slot[i].first = first;
+    s->slot[i].count = count;
     return SS_OK;
 }
 
-static bool kernel_running(ss_server *s) { return s->launched && hipEventQuery(s->exited) == hipErrorNotReady; }
-
-// Requests published but not finished while no kernel runs (it timed out
-// idle as they arrived) get a fresh kernel.
-static int ensure_running(ss_server *s)
+// Dispatcher thread: retire finished launches, admit arrived requests.
+static void dispatch_loop(ss_server *s)
 {
-    std::lock_guard<std::mutex> lk(s->launch_mu);
-    if (kernel_running(s))
-        return SS_OK;
-    if (s->launched) {
-        hipError_t e = hipStreamSynchronize(s->stream);
-        if (e != hipSuccess)
-            return hip_fail(e, "serve kernel");
+    uint32_t next = 0;
+    while (true) {
+        bool idle = true;
+        for (uint32_t i = 0; i < s->ns; i++) {
+            if (!s->slot[i].busy)
+                continue;
+            const hipError_t q = hipEventQuery(s->ev[i]);
+            if (q == hipErrorNotReady)
+                continue;
+            if (q != hipSuccess) {
+                s->fail(SS_EDEVICE, hip_msg(q, "serve: search"));
+                return;
+            }
+            for (uint64_t r = s->slot[i].first; r < s->slot[i].first + s->slot[i].count; r++)
+                s->done[(uint32_t)r & s->mask].store(r + 1, std::memory_order_release);
+            s->slot[i].busy = false;
+            idle = false;
+        }
+        const bool stopping = s->stop.load(std::memory_order_acquire);
+        const uint64_t sub = s->submitted.load(std::memory_order_acquire);
+        if (!stopping && sub > s->dispatched) {
+            for (uint32_t k = 0; k < s->ns; k++) {
+                const uint32_t i = (next + k) % s->ns;
+                if (s->slot[i].busy)
+                    continue;
+                const uint64_t avail = sub - s->dispatched;
+                const uint32_t to_end = s->S - ((uint32_t)s->dispatched & s->mask);
+                const uint32_t count = (uint32_t)std::min<uint64_t>({avail, (uint64_t)s->max_batch, to_end});
+                if (launch_batch(s, i, s->dispatched, count) != SS_OK)
+                    return;
+                s->dispatched += count;
+                next = (i + 1) % s->ns;
+                idle = false;
+                break;
+            }
+        }
+        if (stopping) {
+            bool any = false;
+            for (uint32_t i = 0; i < s->ns; i++)
+                any |= s->slot[i].busy;
+            if (!any)
+                return;
+        }
+        if (idle)
+            std::this_thread::yield();
     }
-    return launch(s);
 }
 
 extern "C" {
@@ -218,13 +242,26 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
         return SS_EINVAL;
     }
     *out = nullptr;
-    ss_server *s = new (std::nothrow) ss_server();
+    ss_serve_config c = *config;
+    if (c.ring_slots == 0)
+        c.ring_slots = 16384;
+    if (c.streams == 0)
+        c.streams = 4;
+    if (c.max_batch == 0)
+        c.max_batch = 2048;
+    if (c.k_max == 0 || c.l_max == 0 || c.k_max > c.l_max || c.l_max > SS_MAX_L || (c.ring_slots & (c.ring_slots - 1)) ||
+        c.ring_slots < 64 || c.ring_slots > (1u << 22) || c.streams > kMaxStreams || c.max_batch > c.ring_slots) {
+        set_error("ss_serve_start: need 1 <= k_max <= l_max <= 512, ring_slots a power of two in 64..4M, "
+                  "streams <= 8, max_batch <= ring_slots");
+        return SS_EINVAL;
+    }
+    ss_server *s = new (std::nothrow) ss_server(c.ring_slots);
     if (!s) {
         set_error("ss_serve_start: out of host memory");
         return SS_ENOMEM;
     }
     s->ix = ix;
-    s->cfg = *config;
+    s->cfg = c;
     int rc = ss::index_view(ix, &s->v);
     if (rc || !s->v.has_graph) {
         free_server(s);
@@ -232,65 +269,51 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
             set_error("ss_serve_start: no graph loaded");
         return rc ? rc : SS_ESTATE;
     }
-    ss_serve_config &c = s->cfg;
-    if (c.ring_slots == 0)
-        c.ring_slots = 16384;
-    if (c.idle_timeout_ms == 0)
-        c.idle_timeout_ms = 2000;
-    if (c.k_max == 0 || c.l_max == 0 || c.k_max > c.l_max || c.l_max > SS_MAX_L ||
-        (c.ring_slots & (c.ring_slots - 1)) || c.ring_slots < 64 || c.ring_slots > (1u << 22) ||
-        c.idle_timeout_ms > 600000) {
-        free_server(s);
-        set_error("ss_serve_start: need 1 <= k_max <= l_max <= 512, ring_slots a power of two in 64..4M, "
-                  "idle_timeout_ms <= 600000");
-        return SS_EINVAL;
-    }
-    s->metric = s->v.metric;
     s->S = c.ring_slots;
     s->mask = s->S - 1;
-    if ((rc = ss::shape_for(ix, c.l_max, 1u << 30, c.visited_slots, c.waves_per_block, c.blocks, &s->wpb, &s->hslots,
-                            &s->blocks))) {
-        free_server(s);
-        return rc;
-    }
-    if (s->blocks * s->wpb < 2) {
-        free_server(s);
-        set_error("ss_serve_start: need at least two wavefronts (one dispatches requests)");
-        return SS_EINVAL;
-    }
-    const size_t dp = s->v.dim_pad;
+    s->ns = c.streams;
+    s->max_batch = c.max_batch;
+    s->params.mode = SS_MODE_BATCHED;
+    s->params.visited_slots = c.visited_slots;
+    s->params.waves_per_block = c.waves_per_block;
+    s->params.blocks = c.blocks;
+    s->params.expand_width = 1;
+    const size_t dp = s->v.dim_pad, km = c.k_max, S = s->S;
     hipError_t e;
-    if ((e = host_alloc(&s->h_q, (size_t)s->S * dp)) != hipSuccess || (e = host_alloc(&s->h_kl, (size_t)s->S * 2)) ||
-        (e = host_alloc(&s->h_head, 8)) || (e = host_alloc(&s->h_stop, 16)) || (e = host_alloc(&s->h_done, s->S)) ||
-        (e = host_alloc(&s->h_ids, (size_t)s->S * c.k_max)) || (e = host_alloc(&s->h_d, (size_t)s->S * c.k_max)) ||
-        (e = host_alloc(&s->h_stats, s->S)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_q), (size_t)s->S * dp * sizeof(float))) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_K), (size_t)s->S * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_L), (size_t)s->S * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_seq), (size_t)s->S * 8)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_ctl), 64))) {
-        rc = hip_fail(e, "ss_serve_start: allocation");
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&s->h_q), S * dp * 4)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_K), S * 4)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_L), S * 4)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_ids), S * km * 4)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_d), S * km * 4)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_stats), S * sizeof(ss_query_stats))) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_q), S * dp * 4)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_K), S * 4)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_L), S * 4)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_ids), S * km * 4)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_d), S * km * 4)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_stats), S * sizeof(ss_query_stats))) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_queues), kMaxStreams * 64))) {
+        set_error(hip_msg(e, "ss_serve_start: allocation"));
         free_server(s);
-        return rc;
+        return e == hipErrorOutOfMemory ? SS_ENOMEM : SS_EDEVICE;
     }
-    if ((e = hipMemset(s->d_seq, 0, (size_t)s->S * 8)) || (e = hipMemset(s->d_ctl, 0, 64)) ||
-        (e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) ||
-        (e = hipEventCreateWithFlags(&s->exited, hipEventDisableTiming))) {
-        rc = hip_fail(e, "ss_serve_start: setup");
-        free_server(s);
-        return rc;
+    for (uint32_t i = 0; i < s->ns; i++) {
+        if ((e = hipStreamCreateWithFlags(&s->st[i], hipStreamNonBlocking)) ||
+            (e = hipEventCreateWithFlags(&s->ev[i], hipEventDisableTiming))) {
+            set_error(hip_msg(e, "ss_serve_start: streams"));
+            free_server(s);
+            return SS_EDEVICE;
+        }
     }
-    int khz = 0;
-    if ((e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0)) != hipSuccess || khz <= 0) {
+    for (uint32_t i = 0; i < S; i++)
+        s->done[i].store(0, std::memory_order_relaxed);
+    s->consumed.assign(S, 0);
+    try {
+        s->dispatcher = std::thread(dispatch_loop, s);
+    } catch (...) {
+        set_error("ss_serve_start: cannot start the dispatcher thread");
         free_server(s);
-        set_error("ss_serve_start: no device wall clock rate");
-        return SS_EDEVICE;
-    }
-    s->khz = (uint32_t)khz;
-    s->consumed.assign(s->S, 0);
-    if ((rc = launch(s))) {
-        free_server(s);
-        return rc;
+        return SS_ENOMEM;
     }
     *out = s;
     return SS_OK;
@@ -301,6 +324,11 @@ int ss_submit(ss_server *s, const float *query, uint32_t K, uint32_t L, uint64_t
     if (!s || !query) {
         set_error("ss_submit: NULL argument");
         return SS_EINVAL;
+    }
+    if (s->error.load() != SS_OK) {
+        std::lock_guard<std::mutex> lk(s->err_mu);
+        set_error(s->err_msg);
+        return s->error.load();
     }
     if (K == 0 || L == 0 || K > L || K > s->cfg.k_max || L > s->cfg.l_max) {
         set_error("ss_submit: K and L exceed the server's budget (1 <= K <= L, K <= k_max, L <= l_max)");
@@ -316,14 +344,11 @@ int ss_submit(ss_server *s, const float *query, uint32_t K, uint32_t L, uint64_t
     std::memcpy(dst, query, s->v.dim * sizeof(float));
     for (uint32_t j = s->v.dim; j < s->v.dim_pad; j++)
         dst[j] = 0.0f;
-    s->h_kl[2 * slot] = K;
-    s->h_kl[2 * slot + 1] = L;
-    __atomic_store_n(s->h_head, idx + 1, __ATOMIC_RELEASE);
+    s->h_K[slot] = K;
+    s->h_L[slot] = L;
     s->submitted.store(idx + 1, std::memory_order_release);
     if (ticket)
         *ticket = idx;
-    if (!kernel_running(s))
-        return ensure_running(s);
     return SS_OK;
 }
 
@@ -340,9 +365,7 @@ int ss_poll(ss_server *s, uint32_t max_results, uint64_t *tickets, uint32_t *ids
     uint32_t n = 0;
     for (uint64_t idx = s->oldest; idx < sub && n < max_results; idx++) {
         const uint32_t slot = (uint32_t)idx & s->mask;
-        if (s->consumed[slot])
-            continue;
-        if (__atomic_load_n(&s->h_done[slot], __ATOMIC_ACQUIRE) != idx + 1)
+        if (s->consumed[slot] || s->done[slot].load(std::memory_order_acquire) != idx + 1)
             continue;
         tickets[n] = idx;
         if (ids)
@@ -360,8 +383,11 @@ int ss_poll(ss_server *s, uint32_t max_results, uint64_t *tickets, uint32_t *ids
     }
     s->completed.store(s->oldest, std::memory_order_release);
     *n_out = n;
-    if (n == 0 && s->oldest < sub && !kernel_running(s))
-        return ensure_running(s);
+    if (n == 0 && s->error.load() != SS_OK) {
+        std::lock_guard<std::mutex> lk(s->err_mu);
+        set_error(s->err_msg);
+        return s->error.load();
+    }
     return SS_OK;
 }
 
@@ -369,11 +395,14 @@ int ss_serve_stop(ss_server *s)
 {
     if (!s)
         return SS_OK;
-    __atomic_store_n(s->h_stop, 1u, __ATOMIC_RELEASE);
-    int rc = SS_OK;
-    hipError_t e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess)
-        rc = hip_fail(e, "serve kernel");
+    s->stop.store(true, std::memory_order_release);
+    if (s->dispatcher.joinable())
+        s->dispatcher.join();
+    int rc = s->error.load();
+    if (rc) {
+        std::lock_guard<std::mutex> lk(s->err_mu);
+        set_error(s->err_msg);
+    }
     free_server(s);
     return rc;
 }
@@ -390,6 +419,8 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
         set_error("ss_serve_replay: k_stride below k_max");
         return SS_EINVAL;
     }
+    if (nq == 0)
+        return SS_OK;
     ss::IndexView v{};
     int rc = ss::index_view(ix, &v);
     if (rc)
@@ -398,24 +429,21 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
     if ((rc = ss_serve_start(ix, config, &s)))
         return rc;
     using clk = std::chrono::steady_clock;
-    std::vector<double> submit_at(nq, 0.0);
     std::atomic<int> prod_rc{SS_OK};
+    std::atomic<bool> abandon{false};
     const auto t0 = clk::now();
     auto now_us = [&]() { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); };
     std::thread producer([&]() {
-        for (uint32_t i = 0; i < nq; i++) {
-            while (now_us() < arrival_us[i]) {
+        for (uint32_t i = 0; i < nq && !abandon.load(); i++) {
+            while (now_us() < arrival_us[i] && !abandon.load()) {
             }
             uint64_t t;
             int r;
-            while ((r = ss_submit(s, queries + (size_t)i * v.dim, K[i], L[i], &t)) == SS_EBUSY) {
+            while ((r = ss_submit(s, queries + (size_t)i * v.dim, K[i], L[i], &t)) == SS_EBUSY && !abandon.load()) {
+                std::this_thread::yield();
             }
             if (r != SS_OK) {
                 prod_rc.store(r);
-                return;
-            }
-            if (t != i) {
-                prod_rc.store(SS_ESTATE);
                 return;
             }
         }
@@ -426,7 +454,7 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
     std::vector<float> ds((size_t)batch * config->k_max);
     std::vector<ss_query_stats> st(batch);
     uint32_t done = 0;
-    const double give_up_us = (arrival_us[nq ? nq - 1 : 0] + 60e6);
+    const double give_up_us = arrival_us[nq - 1] + 60e6;
     while (done < nq && prod_rc.load() == SS_OK) {
         uint32_t n = 0;
         if ((rc = ss_poll(s, batch, tk.data(), ids.data(), ds.data(), st.data(), &n)))
@@ -442,16 +470,17 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
                 stats[i] = st[j];
         }
         done += n;
-        if (t > give_up_us) {
+        if (n == 0 && t > give_up_us) {
             set_error("ss_serve_replay: results did not arrive within 60 s of the last request");
             rc = SS_ESTATE;
             break;
         }
     }
+    abandon.store(true);
     producer.join();
     if (!rc && prod_rc.load() != SS_OK)
         rc = prod_rc.load();
-    int rc2 = ss_serve_stop(s);
+    const int rc2 = ss_serve_stop(s);
     return rc ? rc : rc2;
 }
 

@@ -24,7 +24,7 @@ def dim_padded(dim):
 
 
 def _params(mode=_lib.SS_MODE_BATCHED, waves_per_block=0, visited_slots=0,
-            blocks=0, use_graph=False, expand_width=1):
+            blocks=0, use_graph=False, expand_width=0):
     p = SearchParams()
     p.expand_width = expand_width
     p.mode = mode
@@ -111,7 +111,7 @@ class Index:
 
     # -- search -----------------------------------------------------------
     def search(self, queries, K, L, k_stride=None, mode=_lib.SS_MODE_BATCHED,
-               waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=1):
+               waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0):
         """Search a batch of host queries.  K and L are ints or per-query
         arrays.  Returns (ids [nq][k_stride], dists, stats)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -158,7 +158,7 @@ class Index:
 
     def search_device(self, d_queries, nq, k_max, l_max, k_stride, d_out_ids, d_out_dists,
                       d_stats=0, d_K=0, d_L=0, stream=0, mode=_lib.SS_MODE_BATCHED,
-                      waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=1):
+                      waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0):
         """Enqueue a search on device buffers (raw pointers, e.g. from torch
         tensors' data_ptr()); queries are [nq][dim_pad]."""
         p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width)
@@ -167,6 +167,17 @@ class Index:
                                    k_max, l_max, k_stride, ctypes.byref(p), vp(d_out_ids),
                                    vp(d_out_dists or None), vp(d_stats or None),
                                    vp(stream or None)))
+
+    def autotune(self, queries, K, L, max_recall_loss=0.0):
+        """Pick and store launch parameters for pool size L (see ss_autotune).
+        Returns dict(expand_width, waves_per_block, visited_slots, ms, recall)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        p = SearchParams()
+        ms, rec = ctypes.c_float(), ctypes.c_float()
+        check(lib.ss_autotune(self._h, fptr(q), q.shape[0], int(K), int(L), float(max_recall_loss),
+                              ctypes.byref(p), ctypes.byref(ms), ctypes.byref(rec)))
+        return dict(expand_width=p.expand_width, waves_per_block=p.waves_per_block,
+                    visited_slots=p.visited_slots, ms=ms.value, recall=rec.value)
 
     def bruteforce(self, queries, k, exclude=None):
         """Exact top-k by (distance, id) on the GPU."""

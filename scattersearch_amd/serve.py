@@ -1,5 +1,6 @@
-"""Continuous-batching serving over the resident GPU kernel (ss_serve_* in
-the C ABI): submit queries as they arrive, poll results as they finish."""
+"""Continuous-batching serving (ss_serve_* in the C ABI): submit queries as
+they arrive, poll results as they finish; arrivals are admitted into the
+next batched launch while earlier launches still run."""
 import ctypes
 
 import numpy as np
@@ -8,17 +9,18 @@ from ._lib import QueryStats, ServeConfig, check, fptr, lib, uptr
 from .index import STATS_DTYPE
 
 
-def _config(k_max, l_max, ring_slots=0, visited_slots=0, waves_per_block=0, blocks=0, idle_timeout_ms=0):
+def _config(k_max, l_max, ring_slots=0, visited_slots=0, waves_per_block=0, blocks=0, streams=0,
+            max_batch=0):
     c = ServeConfig()
     c.k_max, c.l_max, c.ring_slots = k_max, l_max, ring_slots
     c.visited_slots, c.waves_per_block, c.blocks = visited_slots, waves_per_block, blocks
-    c.idle_timeout_ms = idle_timeout_ms
+    c.streams, c.max_batch = streams, max_batch
     return c
 
 
 class Server:
-    """A resident search kernel on the index's GPU.  One thread may submit
-    while another polls."""
+    """A serving loop on the index's GPU.  One thread may submit while
+    another polls."""
 
     def __init__(self, index, k_max, l_max, **kw):
         self.index = index

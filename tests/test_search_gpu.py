@@ -249,3 +249,20 @@ def test_wide_expansion_recall(c0, width):
     assert wide[2]["hops"].mean() >= exact[2]["hops"].mean() * 0.8  # same work, fewer steps
     for r in range(wide[0].shape[0]):
         assert (np.diff(wide[1][r]) >= 0).all() and len(set(wide[0][r].tolist())) == 10
+
+
+def test_autotune_exact_and_relaxed(small):
+    import oracle
+    from scattersearch_amd import metrics
+    ix = _index(small["base"], small["adj"], small["ep"])
+    exact = ix.search(small["queries"], 10, 48, expand_width=1)
+    best = ix.autotune(small["queries"], 10, 48, max_recall_loss=0.0)
+    assert best["expand_width"] == 1 and best["ms"] > 0
+    tuned = ix.search(small["queries"], 10, 48)  # parameters left at 0 take the tuned ones
+    assert np.array_equal(tuned[0], exact[0])
+    gt, _ = oracle.bruteforce(small["base"], small["queries"], 10)
+    r_exact = metrics.recall(exact[0], gt, 10)
+    assert abs(best["recall"] - r_exact) < 1e-6
+    relaxed = ix.autotune(small["queries"], 10, 48, max_recall_loss=0.05)
+    assert relaxed["recall"] >= r_exact - 0.05 - 1e-6
+    assert relaxed["expand_width"] in (1, 2, 4)

@@ -1,6 +1,6 @@
-"""Continuous-batching serving on the resident kernel: every result equals
-the same query's result alone (the batch search and the CPU oracle), in any
-mix of arrivals, K and L."""
+"""Continuous-batching serving: every result equals the same query's result
+alone (the batch search and the CPU oracle), in any mix of arrivals, K and L.
+"""
 import time
 
 import numpy as np
@@ -16,7 +16,13 @@ def _index(d):
     return ix
 
 
+@pytest.mark.skip(reason="the resident persistent-kernel server this exercised hung on the GPU (no output "
+                         "for 180 s) and was removed; serving now runs through batched launches, tested below")
 def test_replay_results_equal_alone(small):
+    pass
+
+
+def test_batched_serving_results_equal_alone(small):
     import oracle
     from scattersearch_amd import serve
     ix = _index(small)
@@ -26,8 +32,7 @@ def test_replay_results_equal_alone(small):
     K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
     L = np.array([rng.integers(max(k, 16), 129) for k in K], dtype=np.uint32)
     arr = serve.poisson_arrivals(nq, 20000, seed=1)
-    ids, ds, lat, st = serve.replay(ix, q, K, L, arr, k_max=32, l_max=128, idle_timeout_ms=1000,
-                                    visited_slots=8192)
+    ids, ds, lat, st = serve.replay(ix, q, K, L, arr, k_max=32, l_max=128, visited_slots=8192)
     assert (lat > 0).all()
     for i in range(nq):
         k = int(K[i])
@@ -38,11 +43,11 @@ def test_replay_results_equal_alone(small):
         assert st["hops"][i] == o_hops and st["ndc"][i] == o_ndc
 
 
-def test_submit_poll_and_idle_restart(small):
+def test_submit_and_poll(small):
     from scattersearch_amd import serve
     ix = _index(small)
     alone, _, _ = ix.search(small["queries"][:8], 10, 48, visited_slots=8192)
-    with serve.Server(ix, 10, 48, idle_timeout_ms=50, visited_slots=8192) as srv:
+    with serve.Server(ix, 10, 48, visited_slots=8192, streams=2) as srv:
         got = {}
         for rnd in range(2):
             tickets = [srv.submit(small["queries"][i], 10, 48) for i in range(8)]
@@ -54,14 +59,14 @@ def test_submit_poll_and_idle_restart(small):
             assert len(got) == 8 * (rnd + 1)
             for j, t in enumerate(tickets):
                 assert np.array_equal(got[t], alone[j])
-            time.sleep(0.3)  # the resident kernel idles out; the next submit restarts it
+            time.sleep(0.05)
 
 
 def test_admission_limits(small):
     from scattersearch_amd import SSError, serve
     from scattersearch_amd._lib import SS_EBUSY, SS_EINVAL
     ix = _index(small)
-    with serve.Server(ix, 10, 48, ring_slots=64, idle_timeout_ms=500) as srv:
+    with serve.Server(ix, 10, 48, ring_slots=64, max_batch=16) as srv:
         with pytest.raises(SSError) as e:
             srv.submit(small["queries"][0], 20, 48)  # K above k_max
         assert e.value.code == SS_EINVAL
@@ -82,7 +87,7 @@ def test_admission_limits(small):
 def test_stop_with_requests_in_flight(small):
     from scattersearch_amd import serve
     ix = _index(small)
-    srv = serve.Server(ix, 10, 48, idle_timeout_ms=500)
+    srv = serve.Server(ix, 10, 48)
     for i in range(32):
         srv.submit(small["queries"][i], 10, 48)
     srv.stop()
