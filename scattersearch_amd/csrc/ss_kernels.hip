@@ -24,7 +24,8 @@
 namespace ss {
 namespace {
 
-constexpr int kMaxChunks = SS_MAX_L / 64;
+constexpr int kPoolChunks = SS_MAX_L / 64;
+constexpr int kMaxChunks = kPoolChunks;
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -74,8 +75,7 @@ __device__ __forceinline__ float team_dist(const float *__restrict__ row, const 
     const float4 *r4 = reinterpret_cast<const float4 *>(row);
     const float4 *q4 = reinterpret_cast<const float4 *>(q);
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    if (tail) {
-        const float4 x = r4[4 * nblk + h];
+    auto tail_block = [&](const float4 x) {
         const float4 y = q4[4 * nblk + h];
         if (METRIC == SS_METRIC_L2) {
             const float d0 = y.x - x.x, d1 = y.y - x.y, d2 = y.z - x.z, d3 = y.w - x.w;
@@ -89,11 +89,8 @@ __device__ __forceinline__ float team_dist(const float *__restrict__ row, const 
             s2 = y.z * x.z;
             s3 = y.w * x.w;
         }
-    }
-#pragma unroll
-    for (int b = 0; b < nblk; b++) {
-        const float4 xa = r4[4 * b + h];
-        const float4 xb = r4[4 * b + 2 + h];
+    };
+    auto block = [&](int b, const float4 xa, const float4 xb) {
         const float4 qa = q4[4 * b + h];
         const float4 qb = q4[4 * b + 2 + h];
         if (METRIC == SS_METRIC_L2) {
@@ -108,6 +105,44 @@ __device__ __forceinline__ float team_dist(const float *__restrict__ row, const 
             s1 = s1 + __builtin_fmaf(qa.y, xa.y, qb.y * xb.y);
             s2 = s2 + __builtin_fmaf(qa.z, xa.z, qb.z * xb.z);
             s3 = s3 + __builtin_fmaf(qa.w, xa.w, qb.w * xb.w);
+        }
+    };
+    if constexpr (NBLK >= 0) {
+        // Issue every 16-byte load of the row before the first use, so the
+        // row's cache lines are fetched in parallel rather than one after
+        // another.
+        float4 xa[NBLK > 0 ? NBLK : 1], xb[NBLK > 0 ? NBLK : 1], xt;
+#pragma unroll
+        for (int b = 0; b < NBLK; b++) {
+            xa[b] = r4[4 * b + h];
+            xb[b] = r4[4 * b + 2 + h];
+        }
+        if (TAILI)
+            xt = r4[4 * NBLK + h];
+        __builtin_amdgcn_sched_barrier(0);
+        if (TAILI)
+            tail_block(xt);
+#pragma unroll
+        for (int b = 0; b < NBLK; b++)
+            block(b, xa[b], xb[b]);
+    } else {
+        // Runtime size: four blocks (eight loads) in flight at a time.
+        if (tail)
+            tail_block(r4[4 * nblk + h]);
+        for (int b0 = 0; b0 < nblk; b0 += 4) {
+            float4 xa[4], xb[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (b0 + j < nblk) {
+                    xa[j] = r4[4 * (b0 + j) + h];
+                    xb[j] = r4[4 * (b0 + j) + 2 + h];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (b0 + j < nblk)
+                    block(b0 + j, xa[j], xb[j]);
         }
     }
     const float part = (s0 + s1) + (s2 + s3);
@@ -351,9 +386,10 @@ __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w,
 // and its jump target nk (:244-252) is the merged position of the first
 // smallest element of A.  Every entry before the cursor is explored, so the
 // next node is the first unexplored entry of the merged pool.
-template <int NBLK, int TAILI, int METRIC>
+template <int NBLK, int TAILI, int METRIC, int MC>
 __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
 {
+    constexpr int kMaxChunks = MC;  // pool chunks of 64 held in registers (L <= 64 * MC)
     QueryState st;
     if (!query_begin<NBLK, TAILI, METRIC>(a, w, q, st))
         return;
@@ -714,7 +750,7 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
     query_end(a, w, q, st);
 }
 
-template <int NBLK, int TAILI, int METRIC, int EW>
+template <int NBLK, int TAILI, int METRIC, int EW, int MC>
 __global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -740,7 +776,7 @@ __global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
         if (q >= a.nq)
             break;
         if constexpr (EW == 1)
-            search_one<NBLK, TAILI, METRIC>(a, w, a.q_first + q);
+            search_one<NBLK, TAILI, METRIC, MC>(a, w, a.q_first + q);
         else
             search_wide<NBLK, TAILI, METRIC, EW>(a, w, a.q_first + q);
     }
@@ -961,36 +997,45 @@ hipError_t dispatch_dim(uint32_t dim_pad, Args &&...args)
 
 template <int NBLK, int TAILI, int METRIC>
 struct SearchLaunch {
-    template <int EW>
+    template <int EW, int MC>
     static hipError_t go(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
         const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, EW) * wpb;
-        hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC, EW>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+        hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC, EW, MC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
         return hipGetLastError();
     }
     static hipError_t run(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
         switch (a.expand_width) {
         case 2:
-            return go<2>(a, blocks, wpb, st);
+            return go<2, kPoolChunks>(a, blocks, wpb, st);
         case 4:
-            return go<4>(a, blocks, wpb, st);
+            return go<4, kPoolChunks>(a, blocks, wpb, st);
         default:
-            return go<1>(a, blocks, wpb, st);
+            // The exact search keeps ceil(L/64) pool chunks in registers.
+            if (a.l_max <= 64)
+                return go<1, 1>(a, blocks, wpb, st);
+            if (a.l_max <= 128)
+                return go<1, 2>(a, blocks, wpb, st);
+            if (a.l_max <= 256)
+                return go<1, 4>(a, blocks, wpb, st);
+            return go<1, kPoolChunks>(a, blocks, wpb, st);
         }
     }
-    template <int EW>
+    template <int EW, int MC>
     static hipError_t attr1(size_t max_lds)
     {
-        return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC, EW>),
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC, EW, MC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
     }
     static hipError_t attr(size_t max_lds)
     {
         hipError_t e;
-        if ((e = attr1<1>(max_lds)) != hipSuccess || (e = attr1<2>(max_lds)) != hipSuccess)
+        if ((e = attr1<1, 1>(max_lds)) != hipSuccess || (e = attr1<1, 2>(max_lds)) != hipSuccess ||
+            (e = attr1<1, 4>(max_lds)) != hipSuccess || (e = attr1<1, kPoolChunks>(max_lds)) != hipSuccess ||
+            (e = attr1<2, kPoolChunks>(max_lds)) != hipSuccess)
             return e;
-        return attr1<4>(max_lds);
+        return attr1<4, kPoolChunks>(max_lds);
     }
 };
 
