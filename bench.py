@@ -183,7 +183,7 @@ def main():
     sweep.sort(key=lambda r: r["L"])
 
     # Autotune launch parameters for this L on a sample (exact results kept).
-    tuned = ix.autotune(queries[:min(nq, 2000)], K, L, max_recall_loss=0.0)
+    tuned = ix.autotune(queries, K, L, max_recall_loss=0.0)
     log(f"autotuned for L={L}: {tuned}")
 
     # Timed region: K steps of one batched launch over all queries.
@@ -270,12 +270,12 @@ def main():
 def serve_sweep(ix, queries, gt10, w, rates, nreq):
     """Online serving (the C4 workload): Poisson arrivals at each rate into
     the continuous-batching loop, K = 10 (80%) or 100 (20%), L uniform in
-    64..512; latency = completion seen by the host minus arrival."""
+    max(64, K)..512; latency = completion seen by the host minus arrival."""
     from scattersearch_amd import metrics, serve
     rng = np.random.default_rng(20261207)
     qi = np.arange(nreq) % queries.shape[0]
     K = np.where(rng.random(nreq) < 0.8, 10, 100).astype(np.uint32)
-    L = rng.integers(64, 513, size=nreq).astype(np.uint32)
+    L = rng.integers(np.maximum(64, K), 513).astype(np.uint32)  # K <= L
     Q = queries[qi]
     out = []
     for r in rates:
@@ -293,7 +293,7 @@ def serve_sweep(ix, queries, gt10, w, rates, nreq):
         out.append(row)
         log(f"serve {row}")
     return {"workload": "continuous batching into batched launches on 4 streams, Poisson arrivals, "
-                        "K=10 (80%) / 100 (20%), L uniform 64..512", "rates": out}
+                        "K=10 (80%) / 100 (20%), L uniform max(64, K)..512", "rates": out}
 
 
 def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):

@@ -108,23 +108,32 @@ __device__ __forceinline__ float team_dist(const float *__restrict__ row, const 
         }
     };
     if constexpr (NBLK >= 0) {
-        // Issue every 16-byte load of the row before the first use, so the
+        // Issue the 16-byte loads of the row (in one group up to 8 blocks, in
+        // two halves beyond, to bound registers) before the first use, so the
         // row's cache lines are fetched in parallel rather than one after
         // another.
-        float4 xa[NBLK > 0 ? NBLK : 1], xb[NBLK > 0 ? NBLK : 1], xt;
-#pragma unroll
-        for (int b = 0; b < NBLK; b++) {
-            xa[b] = r4[4 * b + h];
-            xb[b] = r4[4 * b + 2 + h];
-        }
+        constexpr int G = NBLK <= 8 ? NBLK : (NBLK + 1) / 2;
+        float4 xt;
         if (TAILI)
             xt = r4[4 * NBLK + h];
-        __builtin_amdgcn_sched_barrier(0);
-        if (TAILI)
-            tail_block(xt);
 #pragma unroll
-        for (int b = 0; b < NBLK; b++)
-            block(b, xa[b], xb[b]);
+        for (int g0 = 0; g0 < NBLK; g0 += G) {
+            float4 xa[G > 0 ? G : 1], xb[G > 0 ? G : 1];
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                if (g0 + j < NBLK) {
+                    xa[j] = r4[4 * (g0 + j) + h];
+                    xb[j] = r4[4 * (g0 + j) + 2 + h];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (TAILI && g0 == 0)
+                tail_block(xt);
+#pragma unroll
+            for (int j = 0; j < G; j++)
+                if (g0 + j < NBLK)
+                    block(g0 + j, xa[j], xb[j]);
+        }
     } else {
         // Runtime size: four blocks (eight loads) in flight at a time.
         if (tail)
@@ -429,16 +438,35 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         if (fresh)
             w.cand_id[lanes_below(fm)] = nb;
         wave_sync();
-        candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
-        st.ndc += nc;
-        wave_sync();
-
         float cd = __builtin_inff();
         uint32_t cid = kNone;
-        if (lane < nc) {
-            cd = w.cand_d[lane];
-            cid = w.cand_id[lane];
+        if (nc <= 32) {
+            // One pass: the team of lanes 2c, 2c+1 computes candidate c, and
+            // lane c collects its id and distance from lane 2c.
+            const uint32_t c = lane >> 1;
+            uint32_t tid = kNone;
+            float td = __builtin_inff();
+            if (c < nc) {
+                tid = w.cand_id[c];
+                td = team_dist<NBLK, TAILI, METRIC>(a.base + (size_t)tid * a.dim_pad, w.qry, (int)(lane & 1),
+                                                    nblk_rt, tail_rt);
+            }
+            const float gd = __shfl(td, (int)(2 * (lane & 31)), 64);
+            const uint32_t gid = (uint32_t)__shfl((int)tid, (int)(2 * (lane & 31)), 64);
+            if (lane < nc) {
+                cd = gd;
+                cid = gid;
+            }
+        } else {
+            candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
+            wave_sync();
+            if (lane < nc) {
+                cd = w.cand_d[lane];
+                cid = w.cand_id[lane];
+            }
         }
+        st.ndc += nc;
+        wave_sync();
         bool inA = lane < nc && cd < tail;  // strict tail filter, :238
 
         const uint32_t nch = (t + 63u) >> 6;
@@ -750,8 +778,17 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
     query_end(a, w, q, st);
 }
 
+// Minimum wavefronts per SIMD the register budget is sized for.  The 128-d
+// and runtime-size bodies fit in 128 VGPRs (four wavefronts per SIMD, 16 per
+// CU, which is also what the LDS of a small pool allows).  The compiler keeps
+// the lane's share of the query in registers across the neighbour loop; for
+// 96-d that needs up to 168 VGPRs and for 200-d up to 256 (whose visited
+// tables limit residency by LDS before registers do).
+template <int NBLK>
+constexpr int search_min_waves() { return (NBLK == 8 || NBLK < 0) ? 4 : (NBLK <= 6 ? 3 : 2); }
+
 template <int NBLK, int TAILI, int METRIC, int EW, int MC>
-__global__ __launch_bounds__(512) void search_kernel(SearchArgs a)
+__global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wave = threadIdx.x >> 6;

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -430,6 +431,7 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
         return rc;
     using clk = std::chrono::steady_clock;
     std::atomic<int> prod_rc{SS_OK};
+    std::string prod_msg;  // written by the producer before it stores prod_rc
     std::atomic<bool> abandon{false};
     const auto t0 = clk::now();
     auto now_us = [&]() { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); };
@@ -443,6 +445,7 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
                 std::this_thread::yield();
             }
             if (r != SS_OK) {
+                prod_msg = "ss_serve_replay: request " + std::to_string(i) + ": " + ss_last_error();
                 prod_rc.store(r);
                 return;
             }
@@ -478,8 +481,10 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
     }
     abandon.store(true);
     producer.join();
-    if (!rc && prod_rc.load() != SS_OK)
+    if (!rc && prod_rc.load() != SS_OK) {
         rc = prod_rc.load();
+        set_error(prod_msg);  // the message was recorded on the producer thread
+    }
     const int rc2 = ss_serve_stop(s);
     return rc ? rc : rc2;
 }
