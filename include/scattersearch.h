@@ -173,11 +173,13 @@ int ss_autotune(ss_index *index, const float *queries, uint32_t nq, uint32_t K, 
 /* ---------------------------------------------------------------------- */
 /* Continuous-batching serving (tests/online_parallel_search.cpp:78-199,    */
 /* tests/online_pipeline_search.cpp:136-172).  Requests go into a ring in   */
-/* pinned host memory; a dispatcher thread admits everything that has      */
-/* arrived into the next batched launch as soon as one of `streams` launch  */
-/* slots frees up, so requests arriving while others run overlap with them  */
-/* on the GPU; graph and vectors stay resident.  One thread may submit      */
-/* while another polls.                                                     */
+/* coherent pinned host memory; the wavefronts of a running serving         */
+/* activation claim them one at a time, so a request that arrives while     */
+/* others are on the GPU is admitted as soon as a wavefront is free, runs   */
+/* beside them, and completes on its own (result and done flag written into */
+/* the ring by the GPU).  A dispatcher thread starts a new activation when  */
+/* a published request has no owner.  Graph and vectors stay resident.  One */
+/* thread may submit while another polls.                                   */
 /* ---------------------------------------------------------------------- */
 typedef struct ss_server ss_server;
 
@@ -185,11 +187,13 @@ typedef struct ss_serve_config {
     uint32_t k_max;           /* largest K a request may ask for */
     uint32_t l_max;           /* largest L (sizes each wavefront's LDS pool) */
     uint32_t ring_slots;      /* most requests in flight (power of two); 0 = 16384 */
-    uint32_t visited_slots;   /* per-query visited hash; 0 = tuned */
-    uint32_t waves_per_block; /* 0 = tuned */
-    uint32_t blocks;          /* grid cap per launch; 0 = fill the GPU */
-    uint32_t streams;         /* launches in flight at once, 1..8; 0 = 4 */
-    uint32_t max_batch;       /* most requests per launch; 0 = 2048 */
+    uint32_t visited_slots;   /* per-query visited hash; 0 = tuned or default */
+    uint32_t waves_per_block; /* 0 = tuned or default */
+    uint32_t blocks;          /* blocks per activation; 0 = what the LDS holds, <= 8 waves per CU */
+    uint32_t streams;         /* activations alive at once, 1..8; 0 = 2 */
+    uint32_t linger_us;       /* how long a standby wavefront waits for work, <= 200000; 0 = 20000 */
+    uint32_t admit_us;        /* an unclaimed request this old starts another activation; 0 = 50 */
+    uint32_t standby_waves;   /* idle wavefronts kept waiting for work; 0 = 512 */
 } ss_serve_config;
 
 int ss_serve_start(ss_index *index, const ss_serve_config *config, ss_server **out);
@@ -204,7 +208,8 @@ int ss_submit(ss_server *server, const float *query, uint32_t K, uint32_t L, uin
 int ss_poll(ss_server *server, uint32_t max_results, uint64_t *tickets, uint32_t *ids,
             float *dists, ss_query_stats *stats, uint32_t *n_out);
 
-/* Stop the resident kernel and free the server; unpolled results are lost. */
+/* Stop: idle wavefronts leave, running searches finish, then the server is
+ * freed; unpolled results and unclaimed requests are dropped. */
 int ss_serve_stop(ss_server *server);
 
 /* Open-loop load test: starts a server, submits query i at arrival_us[i]

@@ -58,7 +58,9 @@ class ServeConfig(ctypes.Structure):
         ("waves_per_block", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
         ("streams", ctypes.c_uint32),
-        ("max_batch", ctypes.c_uint32),
+        ("linger_us", ctypes.c_uint32),
+        ("admit_us", ctypes.c_uint32),
+        ("standby_waves", ctypes.c_uint32),
     ]
 
 

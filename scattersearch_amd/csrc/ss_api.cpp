@@ -480,15 +480,6 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
 }  // extern "C"
 
 namespace ss {
-struct IndexView {
-    const float *d_base;
-    const uint32_t *d_adj;
-    uint32_t dim, dim_pad, num, r_pad, ep;
-    int metric, num_cus;
-    size_t lds_per_cu;
-    bool has_graph;
-};
-
 int index_view(const ss_index *ix, IndexView *v)
 {
     if (!ix || !v) {
@@ -504,14 +495,11 @@ int index_view(const ss_index *ix, IndexView *v)
     v->ep = ix->ep;
     v->metric = ix->metric;
     v->num_cus = ix->num_cus;
+    v->device = ix->device;
     v->lds_per_cu = ix->lds_per_cu;
     v->has_graph = ix->has_graph;
     return SS_OK;
 }
-
-int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
-                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
-                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue);
 
 int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,
               uint32_t blocks, uint32_t *wpb, uint32_t *slots, uint32_t *blocks_out)
@@ -1022,14 +1010,3 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
 
 }  // extern "C"
 
-namespace ss {
-// A batched search on caller-owned device buffers with its own work-queue
-// counter, so that several searches can run at once on different streams.
-int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
-                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
-                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue)
-{
-    return run_search(ix, d_queries, nq, d_K, d_L, k_max, l_max, k_stride, p, d_ids, d_dists, d_stats, st, nullptr, 0,
-                      d_queue);
-}
-}  // namespace ss

@@ -40,6 +40,34 @@ struct SearchArgs {
     uint32_t expand_width; // 1 (exact), 2 or 4 pool entries expanded per step
 };
 
+// A serving activation: wavefronts claim published requests from a ring in
+// host memory, search them with the exact kernel body and publish each
+// result on its own.  search.queries/Ks/Ls/out_ids/out_dists/stats point
+// into the ring (indexed by ring slot); search.k_stride is the ring's k_max.
+struct ServeCtl {        // device memory, one 64-byte block per server
+    uint64_t head;       // requests claimed
+    uint64_t seen_tail;  // largest `tail` read from host memory so far
+    uint64_t refreshed;  // wall clock of the last read of host memory
+    uint32_t standby;    // wavefronts waiting for work
+    uint32_t stop;       // copy of the host's stop flag
+    uint64_t pad[4];
+};
+
+struct ServeArgs {
+    SearchArgs search;
+    const uint64_t *tail;   // host: number of requests published
+    const uint32_t *stop;   // host: nonzero once the server is stopping
+    ServeCtl *ctl;          // device
+    uint64_t *claimed;      // host: per slot, ticket + 1 once a wavefront owns it
+    uint64_t *done;         // host: per slot, ticket + 1 once its result is written
+    uint32_t mask;          // ring slots - 1
+    uint32_t max_idle_polls;
+    uint32_t max_standby;   // idle wavefronts kept per server; others leave
+    uint32_t pad;
+    uint64_t linger_ticks;  // device wall-clock ticks a standby wavefront waits for work
+    uint64_t refresh_ticks; // least wall-clock ticks between reads of host memory
+};
+
 struct BruteArgs {
     const float *base;
     const float *queries;
@@ -60,6 +88,8 @@ size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash
 // Launchers (return hipError_t of the launch).
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
                          uint32_t waves_per_block, hipStream_t stream);
+hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t waves_per_block,
+                        hipStream_t stream);
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream);
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream);
 hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t dim,
@@ -72,4 +102,22 @@ hipError_t init_kernel_attributes(size_t max_lds);
 
 void set_error(const std::string &msg);
 
+}  // namespace ss
+
+// What the serving loop needs to know of an index (ss_api.cpp owns ss_index).
+struct ss_index;
+namespace ss {
+struct IndexView {
+    const float *d_base;
+    const uint32_t *d_adj;
+    uint32_t dim, dim_pad, num, r_pad, ep;
+    int metric, num_cus, device;
+    size_t lds_per_cu;
+    bool has_graph;
+};
+int index_view(const ss_index *ix, IndexView *v);
+// Launch shape (waves per block, visited slots, blocks) the batched search
+// would use for nq queries with pool bound l_max; 0 arguments pick defaults.
+int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,
+              uint32_t blocks, uint32_t *wpb, uint32_t *slots, uint32_t *blocks_out);
 }  // namespace ss

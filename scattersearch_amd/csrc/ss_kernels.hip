@@ -291,6 +291,46 @@ __device__ __forceinline__ void candidate_distances(const SearchArgs &a, const W
     }
 }
 
+// Request and result words.  RING: the words live in a request ring in host
+// memory that the host writes and reads while the kernel runs; they are
+// accessed with system-scope atomics, which go to memory (no cached copy can
+// be stale and no cache maintenance is needed).
+template <bool RING>
+__device__ __forceinline__ uint32_t ld_word(const uint32_t *p)
+{
+    if constexpr (RING)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        return *p;
+}
+
+template <bool RING>
+__device__ __forceinline__ void st_word(uint32_t *p, uint32_t v)
+{
+    if constexpr (RING)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *p = v;
+}
+
+template <bool RING>
+__device__ __forceinline__ void st_stats(ss_query_stats *p, const ss_query_stats &s)
+{
+    if constexpr (RING) {
+        uint32_t *w = reinterpret_cast<uint32_t *>(p);
+        st_word<true>(w + 0, s.ndc);
+        st_word<true>(w + 1, s.hops);
+        st_word<true>(w + 2, s.lossy);
+        st_word<true>(w + 3, s.reserved);
+        st_word<true>(w + 4, (uint32_t)s.t_start);
+        st_word<true>(w + 5, (uint32_t)(s.t_start >> 32));
+        st_word<true>(w + 6, (uint32_t)s.t_end);
+        st_word<true>(w + 7, (uint32_t)(s.t_end >> 32));
+    } else {
+        *p = s;
+    }
+}
+
 // Per-query state shared by the search variants.
 struct QueryState {
     uint32_t K, L, t, ndc, hops, lossy;
@@ -304,33 +344,42 @@ struct QueryState {
 // point, marked visited (ep itself is neither pooled nor marked),
 // src/annsearch.cpp:185-215.  Returns false (after writing an empty result)
 // for a request the host should have rejected.
-template <int NBLK, int TAILI, int METRIC>
+template <int NBLK, int TAILI, int METRIC, bool RING = false>
 __device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st)
 {
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
     st.t_start = (uint64_t)wall_clock64();
-    st.K = uniform(a.Ks ? a.Ks[q] : a.k_default);
-    st.L = uniform(a.Ls ? a.Ls[q] : a.l_default);
+    st.K = uniform(a.Ks ? ld_word<RING>(a.Ks + q) : a.k_default);
+    st.L = uniform(a.Ls ? ld_word<RING>(a.Ls + q) : a.l_default);
     st.out_ids = a.out_ids + (size_t)q * a.k_stride;
     st.out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
     st.lossy = 0;
     st.hops = 0;
     if (st.K == 0 || st.L == 0 || st.K > st.L || st.L > a.l_max || st.K > a.k_stride) {
         for (uint32_t i = lane; i < a.k_stride; i += 64) {
-            st.out_ids[i] = kNone;
+            st_word<RING>(st.out_ids + i, kNone);
             if (st.out_d)
-                st.out_d[i] = __builtin_inff();
+                st_word<RING>(reinterpret_cast<uint32_t *>(st.out_d) + i, 0x7F800000u);  // +inf
         }
         if (a.stats && lane == 0)
-            a.stats[q] = ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, st.t_start, (uint64_t)wall_clock64()};
+            st_stats<RING>(a.stats + q,
+                           ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, st.t_start, (uint64_t)wall_clock64()});
         return false;
     }
-    const float4 *qsrc = reinterpret_cast<const float4 *>(a.queries + (size_t)q * a.dim_pad);
-    float4 *qdst = reinterpret_cast<float4 *>(w.qry);
-    for (uint32_t i = lane; i < a.dim_pad / 4; i += 64)
-        qdst[i] = qsrc[i];
+    if constexpr (RING) {
+        // dim_pad is a multiple of 8: 8-byte words
+        const uint64_t *qsrc = reinterpret_cast<const uint64_t *>(a.queries + (size_t)q * a.dim_pad);
+        uint64_t *qdst = reinterpret_cast<uint64_t *>(w.qry);
+        for (uint32_t i = lane; i < a.dim_pad / 2; i += 64)
+            qdst[i] = __hip_atomic_load(qsrc + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+        const float4 *qsrc = reinterpret_cast<const float4 *>(a.queries + (size_t)q * a.dim_pad);
+        float4 *qdst = reinterpret_cast<float4 *>(w.qry);
+        for (uint32_t i = lane; i < a.dim_pad / 4; i += 64)
+            qdst[i] = qsrc[i];
+    }
     uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
     for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
         h4[i] = make_uint4(kNone, kNone, kNone, kNone);
@@ -364,14 +413,16 @@ __device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &
 
 // indices[i] = retset[i].id, i < K (src/annsearch.cpp:254-257); distances
 // as the pipeline form returns them (:319-323); counters and timestamps.
+template <bool RING = false>
 __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w, uint32_t q, const QueryState &st)
 {
     const uint32_t lane = lane_id();
     for (uint32_t i = lane; i < a.k_stride; i += 64) {
         const bool ok = i < st.K && i < st.t;
-        st.out_ids[i] = ok ? (w.pool_id[i] & ~kExplored) : kNone;
+        st_word<RING>(st.out_ids + i, ok ? (w.pool_id[i] & ~kExplored) : kNone);
         if (st.out_d)
-            st.out_d[i] = ok ? w.pool_d[i] : __builtin_inff();
+            st_word<RING>(reinterpret_cast<uint32_t *>(st.out_d) + i,
+                          __float_as_uint(ok ? w.pool_d[i] : __builtin_inff()));
     }
     if (a.path)
         for (uint32_t i = st.hops + lane; i < a.path_cap; i += 64)
@@ -381,7 +432,7 @@ __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w,
     for (int off = 32; off > 0; off >>= 1)
         lsum += __shfl_xor(lsum, off, 64);
     if (a.stats && lane == 0)
-        a.stats[q] = ss_query_stats{st.ndc, st.hops, lsum, 0u, st.t_start, (uint64_t)wall_clock64()};
+        st_stats<RING>(a.stats + q, ss_query_stats{st.ndc, st.hops, lsum, 0u, st.t_start, (uint64_t)wall_clock64()});
     wave_sync();
 }
 
@@ -395,12 +446,12 @@ __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w,
 // and its jump target nk (:244-252) is the merged position of the first
 // smallest element of A.  Every entry before the cursor is explored, so the
 // next node is the first unexplored entry of the merged pool.
-template <int NBLK, int TAILI, int METRIC, int MC>
+template <int NBLK, int TAILI, int METRIC, int MC, bool RING = false>
 __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
 {
     constexpr int kMaxChunks = MC;  // pool chunks of 64 held in registers (L <= 64 * MC)
     QueryState st;
-    if (!query_begin<NBLK, TAILI, METRIC>(a, w, q, st))
+    if (!query_begin<NBLK, TAILI, METRIC, RING>(a, w, q, st))
         return;
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
@@ -594,7 +645,7 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
     }
     st.t = t;
 
-    query_end(a, w, q, st);
+    query_end<RING>(a, w, q, st);
 }
 
 // Wider expansion (the ModifiedDeltaStepping idea, src/annsearch.cpp:
@@ -820,6 +871,140 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
 }
 
 __global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
+
+// ---------------------------------------------------------------------------
+// Serving activation.  Requests are published by the host (ring slots filled,
+// then `tail` raised in host memory).  A wavefront claims the next request by
+// advancing ctl.head with a compare-and-swap against the largest tail seen so
+// far (ctl.seen_tail, device memory), runs the exact search on it and
+// publishes its result with a per-slot flag, so every request completes on
+// its own.  Host memory is read for new requests at most once per
+// refresh_ticks across the whole activation: a wavefront that finds no work
+// takes the refresh turn with a compare-and-swap on ctl.refreshed and copies
+// the host's tail (and stop flag) into ctl.  At most max_standby wavefronts
+// wait for work at a time (counted in ctl.standby); the others leave when
+// idle.  The n-th standby wavefront polls ctl every 1 + n/64 sleeps, so the
+// standby pool as a whole polls a few tens of millions of times a second
+// however large it is.  A standby wavefront leaves after linger_ticks of wall
+// clock (or max_idle_polls polls) without work, or when the host's stop flag
+// is seen.
+// No wavefront waits for another and every loop is bounded: the claim loop by
+// the number of published requests (a failed swap means another wavefront
+// claimed one), the idle loop by the poll count.
+//
+// Ordering: every word of the ring (request vector, K, L, result ids and
+// distances, stats, claim and done flags) is read and written with
+// system-scope atomics, which go to host memory without a cached copy, so no
+// cache invalidation or write-back is needed per request.  A request is read
+// only after a read of `tail` showed it published (the host writes the slot
+// before `tail`); the done flag is written only after the wavefront has
+// waited for all of its result stores to complete.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoTicket = ~0ull;
+
+__device__ __forceinline__ uint64_t claim_ticket(ServeCtl *ctl)
+{
+    uint64_t h = __hip_atomic_load(&ctl->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        const uint64_t t = __hip_atomic_load(&ctl->seen_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h >= t)
+            return kNoTicket;
+        if (__hip_atomic_compare_exchange_strong(&ctl->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return h;
+    }
+}
+
+// One read of the host's tail and stop flag, if this wavefront wins the
+// refresh turn (at most one per refresh_ticks).
+__device__ __forceinline__ void refresh_from_host(const ServeArgs &a)
+{
+    const uint64_t now = (uint64_t)wall_clock64();
+    uint64_t last = __hip_atomic_load(&a.ctl->refreshed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (now - last < a.refresh_ticks)
+        return;
+    if (!__hip_atomic_compare_exchange_strong(&a.ctl->refreshed, &last, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+        return;
+    const uint64_t t = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint32_t stop = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_fetch_max(&a.ctl->seen_tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (stop)
+        __hip_atomic_store(&a.ctl->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int NBLK, int TAILI, int METRIC, int MC>
+__global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(ServeArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t wave = threadIdx.x >> 6;
+    const SearchArgs &sa = a.search;
+    const size_t per_wave = wave_lds_bytes(sa.dim_pad, sa.l_max, sa.hash_slots, 1);
+    const WaveLds w = carve(smem + per_wave * wave, sa.dim_pad, sa.l_max, 1);
+    const uint32_t lane = lane_id();
+    bool standby = false;  // counted in ctl.standby
+    uint64_t idle_since = 0;
+    uint32_t polls = 0, sleeps = 1;
+    while (true) {
+        uint64_t tk = kNoTicket;
+        if (lane == 0) {
+            tk = claim_ticket(a.ctl);
+            if (tk == kNoTicket) {
+                refresh_from_host(a);
+                tk = claim_ticket(a.ctl);
+            }
+        }
+        tk = uniform64(tk);
+        if (tk != kNoTicket) {
+            if (standby && lane == 0)
+                __hip_atomic_fetch_add(&a.ctl->standby, 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            standby = false;
+            const uint32_t slot = (uint32_t)tk & a.mask;
+            if (lane == 0)
+                __hip_atomic_store(a.claimed + slot, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            search_one<NBLK, TAILI, METRIC, MC, true>(sa, w, slot);
+            // every result store of the wavefront has completed before the flag
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            wave_sync();
+            if (lane == 0)
+                __hip_atomic_store(a.done + slot, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            continue;
+        }
+        if (!standby) {
+            uint32_t n = 0;
+            if (lane == 0)
+                n = __hip_atomic_fetch_add(&a.ctl->standby, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            n = uniform(n);
+            if (n >= a.max_standby) {  // enough wavefronts are waiting already
+                if (lane == 0)
+                    __hip_atomic_fetch_add(&a.ctl->standby, 0xFFFFFFFFu, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            standby = true;
+            idle_since = (uint64_t)wall_clock64();
+            polls = 0;
+            sleeps = 1u + (n >> 6 < 15u ? n >> 6 : 15u);
+        }
+        const uint32_t stop = uniform(__hip_atomic_load(&a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (stop || ++polls > a.max_idle_polls || (uint64_t)wall_clock64() - idle_since > a.linger_ticks) {
+            if (lane == 0)
+                __hip_atomic_fetch_add(&a.ctl->standby, 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        for (uint32_t i = 0; i < sleeps; i++)
+            __builtin_amdgcn_s_sleep(127);  // about 8,000 cycles each
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Brute force: a block holds kBfQ queries in LDS and streams the base in
@@ -1077,6 +1262,42 @@ struct SearchLaunch {
 };
 
 template <int NBLK, int TAILI, int METRIC>
+struct ServeLaunch {
+    template <int MC>
+    static hipError_t go(const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
+    {
+        const size_t lds = wave_lds_bytes(a.search.dim_pad, a.search.l_max, a.search.hash_slots, 1) * wpb;
+        hipLaunchKernelGGL((serve_kernel<NBLK, TAILI, METRIC, MC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+        return hipGetLastError();
+    }
+    static hipError_t run(const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
+    {
+        const uint32_t l = a.search.l_max;
+        if (l <= 64)
+            return go<1>(a, blocks, wpb, st);
+        if (l <= 128)
+            return go<2>(a, blocks, wpb, st);
+        if (l <= 256)
+            return go<4>(a, blocks, wpb, st);
+        return go<kPoolChunks>(a, blocks, wpb, st);
+    }
+    template <int MC>
+    static hipError_t attr1(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(serve_kernel<NBLK, TAILI, METRIC, MC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
+    static hipError_t attr(size_t max_lds)
+    {
+        hipError_t e;
+        if ((e = attr1<1>(max_lds)) != hipSuccess || (e = attr1<2>(max_lds)) != hipSuccess ||
+            (e = attr1<4>(max_lds)) != hipSuccess)
+            return e;
+        return attr1<kPoolChunks>(max_lds);
+    }
+};
+
+template <int NBLK, int TAILI, int METRIC>
 struct BfLaunch {
     static hipError_t run(const BruteArgs &a, hipStream_t st)
     {
@@ -1124,6 +1345,13 @@ hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint3
     return dispatch_dim<SearchLaunch, SS_METRIC_L2>(a.dim_pad, a, blocks, wpb, stream);
 }
 
+hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t stream)
+{
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<ServeLaunch, SS_METRIC_INNER_PRODUCT>(a.search.dim_pad, a, blocks, wpb, stream);
+    return dispatch_dim<ServeLaunch, SS_METRIC_L2>(a.search.dim_pad, a, blocks, wpb, stream);
+}
+
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream)
 {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, queue);
@@ -1150,6 +1378,10 @@ hipError_t init_kernel_attributes(size_t max_lds)
     if ((e = attr_all<SearchLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<SearchLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<ServeLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<ServeLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<BfLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;

@@ -6,20 +6,21 @@
 // (tests/online_pipeline_search.cpp:136-172); latency is completion minus
 // receive time (online_parallel_search.cpp:154-158).
 //
-// Here submit() drops a request into a ring in pinned host memory.  A
-// dispatcher thread admits every request that has arrived into the next
-// batched search launch as soon as one of `streams` launch slots is free: it
-// copies the new requests' vectors, K and L to the device, runs the batched
-// search (each launch with its own work-queue counter) and copies the results
-// back, all asynchronously on that slot's stream.  So requests that arrive
-// while others are on the GPU start in the next launch and overlap with the
-// ones running, graph and vectors stay resident, and batches form by
-// themselves from whatever arrived during the previous launches.  Admission
-// is by budget: every launch's wavefronts are sized for k_max and l_max (LDS
-// pool and visited hash), so larger K or L are refused, and at most
-// `ring_slots` requests are in flight.  Every kernel launched here ends on its
-// own (no device-side waiting), so nothing the host does can leave the GPU
-// spinning.
+// Here submit() writes a request into a ring in coherent pinned host memory
+// and publishes it by raising `tail`.  On the GPU, the wavefronts of a
+// running serving activation (ss_kernels.hip, serve_kernel) claim published
+// requests one at a time, search them with the exact kernel body while other
+// requests are in flight on the other wavefronts, and write each result and
+// its done flag straight into the ring, so a request completes as soon as its
+// own search ends.  An activation's idle wavefronts keep polling for new work
+// for `linger_us` and then leave; a dispatcher thread starts a new activation
+// (on one of `streams` streams) when a published request has no owner - none
+// is running, or the oldest unclaimed request has waited `admit_us`.  Graph,
+// vectors and the ring stay resident.  Admission is by budget: activations
+// are sized for k_max and l_max (LDS pool and visited hash per wavefront) and
+// for the GPU's LDS, larger K or L are refused, and at most `ring_slots`
+// requests are in flight (SS_EBUSY beyond).  No wavefront waits for another
+// and every device loop is bounded, so every activation ends on its own.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -34,32 +35,35 @@
 
 using ss::set_error;
 
-struct ss_index;
-namespace ss {
-struct IndexView {
-    const float *d_base;
-    const uint32_t *d_adj;
-    uint32_t dim, dim_pad, num, r_pad, ep;
-    int metric, num_cus;
-    size_t lds_per_cu;
-    bool has_graph;
-};
-int index_view(const ss_index *ix, IndexView *v);
-int search_launch(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
-                  uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p, uint32_t *d_ids,
-                  float *d_dists, ss_query_stats *d_stats, hipStream_t st, uint32_t *d_queue);
-}  // namespace ss
 
 namespace {
 
 constexpr uint32_t kMaxStreams = 8;
+// Bound on a standby wavefront's polls whatever the clock says (at least one
+// sleep of some 3.5 us per poll, so at least 0.23 s, above the largest linger).
+constexpr uint32_t kMaxIdlePolls = 65536;
+// Least time between two reads of the host's tail by one activation.
+constexpr uint64_t kRefreshNs = 2000;
 
 std::string hip_msg(hipError_t e, const char *what) { return std::string(what) + ": " + hipGetErrorString(e); }
 
-struct Launch {
-    bool busy = false;
-    uint64_t first = 0, count = 0;
+using clk = std::chrono::steady_clock;
+
+// The part of the ring the GPU reads or writes, in one coherent allocation.
+struct Control {
+    uint64_t tail;  // requests published
+    uint32_t stop;  // nonzero: idle wavefronts leave at once
+    uint32_t pad[13];
 };
+
+template <typename T>
+T *dev_ptr(T *host)
+{
+    void *d = nullptr;
+    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess)
+        return nullptr;
+    return static_cast<T *>(d);
+}
 
 }  // namespace
 
@@ -67,38 +71,32 @@ struct ss_server {
     ss_index *ix = nullptr;
     ss::IndexView v{};
     ss_serve_config cfg{};
-    uint32_t S = 0, mask = 0, ns = 0, max_batch = 0;
-    ss_search_params params{};
-    // pinned host rings
+    uint32_t S = 0, mask = 0, ns = 0, wpb = 0, blocks = 0;
+    // ring in coherent pinned host memory, shared with the GPU
     float *h_q = nullptr;
     uint32_t *h_K = nullptr, *h_L = nullptr;
     uint32_t *h_ids = nullptr;
     float *h_d = nullptr;
     ss_query_stats *h_stats = nullptr;
-    std::vector<std::atomic<uint64_t>> done;  // request index + 1 once its result is on the host
-    // device rings
-    float *d_q = nullptr;
-    uint32_t *d_K = nullptr, *d_L = nullptr;
-    uint32_t *d_ids = nullptr;
-    float *d_d = nullptr;
-    ss_query_stats *d_stats = nullptr;
-    uint32_t *d_queues = nullptr;  // one work-queue counter per launch slot, 64 bytes apart
+    uint64_t *h_claimed = nullptr;  // ticket + 1 once a wavefront owns the slot
+    uint64_t *h_done = nullptr;     // ticket + 1 once the slot's result is written
+    Control *h_ctl = nullptr;
+    std::vector<clk::time_point> t_pub;  // publication time per slot
+    ss::ServeCtl *d_ctl = nullptr;       // claims and the device's copy of tail (device memory)
+    ss::ServeArgs args{};
     hipStream_t st[kMaxStreams] = {};
     hipEvent_t ev[kMaxStreams] = {};
-    Launch slot[kMaxStreams];
+    bool busy[kMaxStreams] = {};
+    uint64_t activations = 0;  // dispatcher thread only
     std::thread dispatcher;
     std::atomic<bool> stop{false};
     std::atomic<int> error{SS_OK};
     std::mutex err_mu;
     std::string err_msg;
-    // producer / consumer counters
     std::atomic<uint64_t> submitted{0};
     std::atomic<uint64_t> completed{0};
-    uint64_t dispatched = 0;  // dispatcher thread only
-    uint64_t oldest = 0;      // poller only
+    uint64_t oldest = 0;  // poller only
     std::vector<uint8_t> consumed;
-
-    explicit ss_server(uint32_t slots) : done(slots) {}
 
     void fail(int code, const std::string &msg)
     {
@@ -114,6 +112,8 @@ static void free_server(ss_server *s)
 {
     if (!s)
         return;
+    // every activation ends by itself (bounded loops); wait for them before
+    // the memory they write goes away
     for (uint32_t i = 0; i < kMaxStreams; i++) {
         if (s->st[i])
             (void)hipStreamSynchronize(s->st[i]);
@@ -122,23 +122,17 @@ static void free_server(ss_server *s)
         if (p)
             (void)hipHostFree(p);
     };
-    auto df = [](void *p) {
-        if (p)
-            (void)hipFree(p);
-    };
     hf(s->h_q);
     hf(s->h_K);
     hf(s->h_L);
     hf(s->h_ids);
     hf(s->h_d);
     hf(s->h_stats);
-    df(s->d_q);
-    df(s->d_K);
-    df(s->d_L);
-    df(s->d_ids);
-    df(s->d_d);
-    df(s->d_stats);
-    df(s->d_queues);
+    hf(s->h_claimed);
+    hf(s->h_done);
+    hf(s->h_ctl);
+    if (s->d_ctl)
+        (void)hipFree(s->d_ctl);
     for (uint32_t i = 0; i < kMaxStreams; i++) {
         if (s->ev[i])
             (void)hipEventDestroy(s->ev[i]);
@@ -148,88 +142,65 @@ static void free_server(ss_server *s)
     delete s;
 }
 
-// Launch requests [first, first+count), which occupy contiguous ring slots.
-static int launch_batch(ss_server *s, uint32_t i, uint64_t first, uint32_t count)
-{
-    const uint32_t s0 = (uint32_t)first & s->mask;
-    const size_t dp = s->v.dim_pad, km = s->cfg.k_max;
-    hipStream_t st = s->st[i];
-    hipError_t e;
-    if ((e = hipMemcpyAsync(s->d_q + s0 * dp, s->h_q + s0 * dp, count * dp * sizeof(float), hipMemcpyHostToDevice,
-                            st)) ||
-        (e = hipMemcpyAsync(s->d_K + s0, s->h_K + s0, count * 4, hipMemcpyHostToDevice, st)) ||
-        (e = hipMemcpyAsync(s->d_L + s0, s->h_L + s0, count * 4, hipMemcpyHostToDevice, st))) {
-        s->fail(SS_EDEVICE, hip_msg(e, "serve: request upload"));
-        return SS_EDEVICE;
-    }
-    int rc = ss::search_launch(s->ix, s->d_q + s0 * dp, count, s->d_K + s0, s->d_L + s0, s->cfg.k_max, s->cfg.l_max,
-                               s->cfg.k_max, &s->params, s->d_ids + s0 * km, s->d_d + s0 * km, s->d_stats + s0, st,
-                               s->d_queues + 16 * i);
-    if (rc) {
-        s->fail(rc, "serve: search launch failed");
-        return rc;
-    }
-    if ((e = hipMemcpyAsync(s->h_ids + s0 * km, s->d_ids + s0 * km, count * km * 4, hipMemcpyDeviceToHost, st)) ||
-        (e = hipMemcpyAsync(s->h_d + s0 * km, s->d_d + s0 * km, count * km * 4, hipMemcpyDeviceToHost, st)) ||
-        (e = hipMemcpyAsync(s->h_stats + s0, s->d_stats + s0, count * sizeof(ss_query_stats), hipMemcpyDeviceToHost,
-                            st)) ||
-        (e = hipEventRecord(s->ev[i], st))) {
-        s->fail(SS_EDEVICE, hip_msg(e, "serve: result download"));
-        return SS_EDEVICE;
-    }
-    s->slot[i].busy = true;
-    s->slot[i].first = first;
-    s->slot[i].count = count;
-    return SS_OK;
-}
+static uint64_t load_acquire(const uint64_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
 
-// Dispatcher thread: retire finished launches, admit arrived requests.
+// Dispatcher thread: starts activations while published requests lack an
+// owner; on stop, tells idle wavefronts to leave and waits for the streams.
 static void dispatch_loop(ss_server *s)
 {
+    (void)hipSetDevice(s->v.device);
+    const auto admit = std::chrono::microseconds(s->cfg.admit_us);
+    uint64_t scan = 0;  // every ticket below scan is known to be claimed
     uint32_t next = 0;
     while (true) {
-        bool idle = true;
+        uint32_t alive = 0;
         for (uint32_t i = 0; i < s->ns; i++) {
-            if (!s->slot[i].busy)
+            if (!s->busy[i])
                 continue;
             const hipError_t q = hipEventQuery(s->ev[i]);
-            if (q == hipErrorNotReady)
+            if (q == hipErrorNotReady) {
+                alive++;
                 continue;
-            if (q != hipSuccess) {
-                s->fail(SS_EDEVICE, hip_msg(q, "serve: search"));
-                return;
             }
-            for (uint64_t r = s->slot[i].first; r < s->slot[i].first + s->slot[i].count; r++)
-                s->done[(uint32_t)r & s->mask].store(r + 1, std::memory_order_release);
-            s->slot[i].busy = false;
-            idle = false;
+            s->busy[i] = false;
+            if (q != hipSuccess) {
+                s->fail(SS_EDEVICE, hip_msg(q, "serve: activation"));
+                __atomic_store_n(&s->h_ctl->stop, 1u, __ATOMIC_RELEASE);
+            }
         }
-        const bool stopping = s->stop.load(std::memory_order_acquire);
-        const uint64_t sub = s->submitted.load(std::memory_order_acquire);
-        if (!stopping && sub > s->dispatched) {
+        if (s->stop.load(std::memory_order_acquire) || s->error.load() != SS_OK) {
+            __atomic_store_n(&s->h_ctl->stop, 1u, __ATOMIC_RELEASE);
+            if (alive == 0)
+                return;
+            std::this_thread::yield();
+            continue;
+        }
+        const uint64_t pub = s->submitted.load(std::memory_order_acquire);
+        // a later claim of the same slot implies this one (claims are in order)
+        while (scan < pub && load_acquire(s->h_claimed + (scan & s->mask)) >= scan + 1)
+            scan++;
+        bool launched = false;
+        if (scan < pub && alive < s->ns &&
+            (alive == 0 || clk::now() - s->t_pub[scan & s->mask] >= admit)) {
             for (uint32_t k = 0; k < s->ns; k++) {
                 const uint32_t i = (next + k) % s->ns;
-                if (s->slot[i].busy)
+                if (s->busy[i])
                     continue;
-                const uint64_t avail = sub - s->dispatched;
-                const uint32_t to_end = s->S - ((uint32_t)s->dispatched & s->mask);
-                const uint32_t count = (uint32_t)std::min<uint64_t>({avail, (uint64_t)s->max_batch, to_end});
-                if (launch_batch(s, i, s->dispatched, count) != SS_OK)
-                    return;
-                s->dispatched += count;
+                hipError_t e = ss::launch_serve(s->v.metric, s->args, s->blocks, s->wpb, s->st[i]);
+                if (e == hipSuccess)
+                    e = hipEventRecord(s->ev[i], s->st[i]);
+                if (e != hipSuccess) {
+                    s->fail(SS_EDEVICE, hip_msg(e, "serve: activation launch"));
+                    break;
+                }
+                s->busy[i] = true;
+                s->activations++;
                 next = (i + 1) % s->ns;
-                idle = false;
+                launched = true;
                 break;
             }
         }
-        if (stopping) {
-            bool any = false;
-            for (uint32_t i = 0; i < s->ns; i++)
-                any |= s->slot[i].busy;
-            if (!any)
-                return;
-        }
-        if (idle)
+        if (!launched)
             std::this_thread::yield();
     }
 }
@@ -247,16 +218,20 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     if (c.ring_slots == 0)
         c.ring_slots = 16384;
     if (c.streams == 0)
-        c.streams = 4;
-    if (c.max_batch == 0)
-        c.max_batch = 2048;
+        c.streams = 2;
+    if (c.linger_us == 0)
+        c.linger_us = 20000;
+    if (c.admit_us == 0)
+        c.admit_us = 50;
+    if (c.standby_waves == 0)
+        c.standby_waves = 512;
     if (c.k_max == 0 || c.l_max == 0 || c.k_max > c.l_max || c.l_max > SS_MAX_L || (c.ring_slots & (c.ring_slots - 1)) ||
-        c.ring_slots < 64 || c.ring_slots > (1u << 22) || c.streams > kMaxStreams || c.max_batch > c.ring_slots) {
+        c.ring_slots < 64 || c.ring_slots > (1u << 22) || c.streams > kMaxStreams || c.linger_us > 200000u) {
         set_error("ss_serve_start: need 1 <= k_max <= l_max <= 512, ring_slots a power of two in 64..4M, "
-                  "streams <= 8, max_batch <= ring_slots");
+                  "streams <= 8, linger_us <= 200 ms");
         return SS_EINVAL;
     }
-    ss_server *s = new (std::nothrow) ss_server(c.ring_slots);
+    ss_server *s = new (std::nothrow) ss_server();
     if (!s) {
         set_error("ss_serve_start: out of host memory");
         return SS_ENOMEM;
@@ -270,33 +245,46 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
             set_error("ss_serve_start: no graph loaded");
         return rc ? rc : SS_ESTATE;
     }
+    // Activation shape: LDS per wavefront for k_max/l_max; by default as
+    // many wavefronts as the LDS holds, at most 8 per CU.  Wavefronts stay
+    // for linger_us after their last request, so the activation keeps its
+    // size under steady load; a second one fills what an activation that
+    // thinned out during a quiet spell gave back.
+    uint32_t wpb = 0, slots = 0, full = 0;
+    if ((rc = ss::shape_for(ix, c.l_max, 1u << 30, c.visited_slots, c.waves_per_block, 0, &wpb, &slots, &full))) {
+        free_server(s);
+        return rc;
+    }
+    const uint32_t dflt = std::max(1u, std::min(full, (8u * (uint32_t)s->v.num_cus + wpb - 1) / wpb));
     s->S = c.ring_slots;
     s->mask = s->S - 1;
     s->ns = c.streams;
-    s->max_batch = c.max_batch;
-    s->params.mode = SS_MODE_BATCHED;
-    s->params.visited_slots = c.visited_slots;
-    s->params.waves_per_block = c.waves_per_block;
-    s->params.blocks = c.blocks;
-    s->params.expand_width = 1;
+    s->wpb = wpb;
+    s->blocks = c.blocks ? std::min(c.blocks, full) : dflt;
     const size_t dp = s->v.dim_pad, km = c.k_max, S = s->S;
+    const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
     hipError_t e;
-    if ((e = hipHostMalloc(reinterpret_cast<void **>(&s->h_q), S * dp * 4)) ||
-        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_K), S * 4)) ||
-        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_L), S * 4)) ||
-        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_ids), S * km * 4)) ||
-        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_d), S * km * 4)) ||
-        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_stats), S * sizeof(ss_query_stats))) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_q), S * dp * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_K), S * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_L), S * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_ids), S * km * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_d), S * km * 4)) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_stats), S * sizeof(ss_query_stats))) ||
-        (e = hipMalloc(reinterpret_cast<void **>(&s->d_queues), kMaxStreams * 64))) {
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&s->h_q), S * dp * 4, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_K), S * 4, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_L), S * 4, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_ids), S * km * 4, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_d), S * km * 4, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_stats), S * sizeof(ss_query_stats), fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_claimed), S * 8, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_done), S * 8, fl)) ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&s->h_ctl), sizeof(Control), fl)) ||
+        (e = hipMalloc(reinterpret_cast<void **>(&s->d_ctl), sizeof(ss::ServeCtl)))) {
         set_error(hip_msg(e, "ss_serve_start: allocation"));
         free_server(s);
         return e == hipErrorOutOfMemory ? SS_ENOMEM : SS_EDEVICE;
+    }
+    std::memset(s->h_claimed, 0, S * 8);
+    std::memset(s->h_done, 0, S * 8);
+    std::memset(s->h_ctl, 0, sizeof(Control));
+    if ((e = hipMemset(s->d_ctl, 0, sizeof(ss::ServeCtl)))) {
+        set_error(hip_msg(e, "ss_serve_start: hipMemset"));
+        free_server(s);
+        return SS_EDEVICE;
     }
     for (uint32_t i = 0; i < s->ns; i++) {
         if ((e = hipStreamCreateWithFlags(&s->st[i], hipStreamNonBlocking)) ||
@@ -306,8 +294,52 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
             return SS_EDEVICE;
         }
     }
-    for (uint32_t i = 0; i < S; i++)
-        s->done[i].store(0, std::memory_order_relaxed);
+    ss::ServeArgs &a = s->args;
+    std::memset(&a, 0, sizeof(a));
+    a.search.base = s->v.d_base;
+    a.search.adj = s->v.d_adj;
+    a.search.queries = dev_ptr(s->h_q);
+    a.search.Ks = dev_ptr(s->h_K);
+    a.search.Ls = dev_ptr(s->h_L);
+    a.search.k_default = c.k_max;
+    a.search.l_default = c.l_max;
+    a.search.nq = s->S;
+    a.search.dim_pad = s->v.dim_pad;
+    a.search.r_pad = s->v.r_pad;
+    a.search.ep = s->v.ep;
+    a.search.l_max = c.l_max;
+    a.search.hash_slots = slots;
+    uint32_t lg = 0;
+    while ((1u << lg) < slots)
+        lg++;
+    a.search.hash_shift = 32u - lg;
+    a.search.k_stride = c.k_max;
+    a.search.out_ids = dev_ptr(s->h_ids);
+    a.search.out_dists = dev_ptr(s->h_d);
+    a.search.stats = dev_ptr(s->h_stats);
+    a.search.expand_width = 1;
+    a.tail = dev_ptr(&s->h_ctl->tail);
+    a.stop = dev_ptr(&s->h_ctl->stop);
+    a.ctl = s->d_ctl;
+    a.claimed = dev_ptr(s->h_claimed);
+    a.done = dev_ptr(s->h_done);
+    a.mask = s->mask;
+    a.max_idle_polls = kMaxIdlePolls;
+    a.max_standby = c.standby_waves;
+    uint32_t khz = 0;
+    if ((rc = ss_wall_clock_khz(ix, &khz))) {
+        free_server(s);
+        return rc;
+    }
+    a.linger_ticks = (uint64_t)khz * c.linger_us / 1000u;
+    a.refresh_ticks = std::max<uint64_t>(1, (uint64_t)khz * kRefreshNs / 1000000u);
+    if (!a.search.queries || !a.search.Ks || !a.search.Ls || !a.search.out_ids || !a.search.out_dists ||
+        !a.search.stats || !a.tail || !a.stop || !a.claimed || !a.done || a.linger_ticks == 0) {
+        set_error("ss_serve_start: cannot map the request ring for the GPU");
+        free_server(s);
+        return SS_EDEVICE;
+    }
+    s->t_pub.assign(S, clk::time_point{});
     s->consumed.assign(S, 0);
     try {
         s->dispatcher = std::thread(dispatch_loop, s);
@@ -347,6 +379,9 @@ int ss_submit(ss_server *s, const float *query, uint32_t K, uint32_t L, uint64_t
         dst[j] = 0.0f;
     s->h_K[slot] = K;
     s->h_L[slot] = L;
+    s->t_pub[slot] = clk::now();
+    // the ring slot is written before the GPU can see the new tail
+    __atomic_store_n(&s->h_ctl->tail, idx + 1, __ATOMIC_RELEASE);
     s->submitted.store(idx + 1, std::memory_order_release);
     if (ticket)
         *ticket = idx;
@@ -366,7 +401,7 @@ int ss_poll(ss_server *s, uint32_t max_results, uint64_t *tickets, uint32_t *ids
     uint32_t n = 0;
     for (uint64_t idx = s->oldest; idx < sub && n < max_results; idx++) {
         const uint32_t slot = (uint32_t)idx & s->mask;
-        if (s->consumed[slot] || s->done[slot].load(std::memory_order_acquire) != idx + 1)
+        if (s->consumed[slot] || load_acquire(s->h_done + slot) != idx + 1)
             continue;
         tickets[n] = idx;
         if (ids)

@@ -1,6 +1,7 @@
 """Continuous-batching serving (ss_serve_* in the C ABI): submit queries as
-they arrive, poll results as they finish; arrivals are admitted into the
-next batched launch while earlier launches still run."""
+they arrive, poll results as they finish; an arriving query is claimed by a
+free wavefront of the running serving kernel and searched beside the ones
+already on the GPU."""
 import ctypes
 
 import numpy as np
@@ -10,11 +11,12 @@ from .index import STATS_DTYPE
 
 
 def _config(k_max, l_max, ring_slots=0, visited_slots=0, waves_per_block=0, blocks=0, streams=0,
-            max_batch=0):
+            linger_us=0, admit_us=0, standby_waves=0):
     c = ServeConfig()
     c.k_max, c.l_max, c.ring_slots = k_max, l_max, ring_slots
     c.visited_slots, c.waves_per_block, c.blocks = visited_slots, waves_per_block, blocks
-    c.streams, c.max_batch = streams, max_batch
+    c.streams, c.linger_us, c.admit_us = streams, linger_us, admit_us
+    c.standby_waves = standby_waves
     return c
 
 

@@ -1,5 +1,6 @@
 """Continuous-batching serving: every result equals the same query's result
-alone (the batch search and the CPU oracle), in any mix of arrivals, K and L.
+alone (the batch search and the CPU oracle), in any mix of arrivals, K and L,
+across ring wrap-around and across serving activations that end and restart.
 """
 import time
 
@@ -16,13 +17,14 @@ def _index(d):
     return ix
 
 
-@pytest.mark.skip(reason="the resident persistent-kernel server this exercised hung on the GPU (no output "
-                         "for 180 s) and was removed; serving now runs through batched launches, tested below")
+@pytest.mark.skip(reason="an earlier resident server (worker wavefronts waiting for a dispatcher wavefront "
+                         "to publish their slot) hung on the GPU (no output for 180 s) and was removed; the "
+                         "current serving kernel never waits on another wavefront and is tested below")
 def test_replay_results_equal_alone(small):
     pass
 
 
-def test_batched_serving_results_equal_alone(small):
+def test_serving_results_equal_alone(small):
     import oracle
     from scattersearch_amd import serve
     ix = _index(small)
@@ -66,7 +68,7 @@ def test_admission_limits(small):
     from scattersearch_amd import SSError, serve
     from scattersearch_amd._lib import SS_EBUSY, SS_EINVAL
     ix = _index(small)
-    with serve.Server(ix, 10, 48, ring_slots=64, max_batch=16) as srv:
+    with serve.Server(ix, 10, 48, ring_slots=64) as srv:
         with pytest.raises(SSError) as e:
             srv.submit(small["queries"][0], 20, 48)  # K above k_max
         assert e.value.code == SS_EINVAL
@@ -91,3 +93,31 @@ def test_stop_with_requests_in_flight(small):
     for i in range(32):
         srv.submit(small["queries"][i], 10, 48)
     srv.stop()
+
+
+def test_ring_wraps_and_activations_restart(small):
+    """300 requests through a 64-slot ring, in bursts separated by pauses
+    longer than an idle wavefront's linger, so slots are reused and new
+    activations start; each result equals the query searched alone."""
+    from scattersearch_amd import serve
+    ix = _index(small)
+    q = small["queries"]
+    alone, alone_d, _ = ix.search(q, 10, 48, visited_slots=4096)
+    got = {}
+    with serve.Server(ix, 10, 48, ring_slots=64, visited_slots=4096, linger_us=500) as srv:
+        for burst in range(6):
+            for i in range(50):
+                t = 50 * burst + i
+                assert srv.submit(q[t % q.shape[0]], 10, 48) == t
+            t0 = time.time()
+            while len(got) < 50 * (burst + 1) and time.time() - t0 < 20:
+                tk, ids, ds, st = srv.poll()
+                for t, r, d in zip(tk, ids, ds):
+                    got[int(t)] = (r.copy(), d.copy())
+            assert len(got) == 50 * (burst + 1), f"burst {burst}"
+            time.sleep(0.005)  # every wavefront has left; the next burst starts a new activation
+    assert len(got) == 300
+    for t, (r, d) in got.items():
+        j = t % q.shape[0]
+        assert np.array_equal(r, alone[j]), f"ticket {t}"
+        assert np.array_equal(d.view(np.uint32), alone_d[j].view(np.uint32))
