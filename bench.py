@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--visited-slots", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--graph-cache", default="",
+                    help="NSG file: load the graph from it if it exists, else build and write it")
     ap.add_argument("--profile", action="store_true",
                     help="only build, then run --steps batched launches (for rocprofv3)")
     ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000",
@@ -77,6 +79,7 @@ def main():
     import torch  # noqa: F401  (shares its HIP runtime with the library)
     import scattersearch_amd as ss
     from scattersearch_amd import data, metrics
+    from scattersearch_amd import io as ssio
     from scattersearch_amd.graph import gpu_build_graph
 
     torch.cuda.set_device(0)
@@ -89,8 +92,17 @@ def main():
 
     ix = ss.Index(base, w.metric)
     t0 = time.time()
-    adj, ep, ginfo = gpu_build_graph(ix, base, w.R, w.knn_k, w.nav_L, log=log)
-    log(f"graph built in {time.time() - t0:.1f}s, ep {ep}")
+    if args.graph_cache and os.path.exists(args.graph_cache):
+        adj, ep = ssio.read_nsg(args.graph_cache, r_pad=w.R)
+        ix.set_graph(adj, ep)
+        ginfo = {"passes": [{"mean_degree": float((adj != 0xFFFFFFFF).sum(1).mean())}], "knn_s": 0.0,
+                 "cached": args.graph_cache}
+        log(f"graph read from {args.graph_cache}, ep {ep}")
+    else:
+        adj, ep, ginfo = gpu_build_graph(ix, base, w.R, w.knn_k, w.nav_L, log=log)
+        log(f"graph built in {time.time() - t0:.1f}s, ep {ep}")
+        if args.graph_cache:
+            ssio.write_nsg(args.graph_cache, adj, ep)
     r_pad = ix.info()["r_pad"]
     K = w.K
 
@@ -260,12 +272,45 @@ def main():
         "l_sweep": sweep,
         "graph": {"mean_degree": ginfo["passes"][-1]["mean_degree"], "knn_s": round(ginfo["knn_s"], 1),
                   "passes": len(ginfo["passes"])},
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": round(launch_bytes), "kernel_ms": round(kern_ms, 4)},
+        "roofline": roofline_record(w.name, nq, K, L, achieved_gbs, launch_bytes, kern_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(workload, nq, K, L):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    calibrated against a plain copy) for this workload and launch, from the
+    record kept in profiles/search_kernel_<workload>_counters.md; None when
+    there is no record for exactly this launch."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"search_kernel_{workload}_counters.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    block = text.split("```json", 1)
+    if len(block) < 2:
+        return None
+    try:
+        rec = json.loads(block[1].split("```", 1)[0])
+    except ValueError:
+        return None
+    if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L")) != (workload, nq, K, L):
+        return None
+    return rec
+
+
+def roofline_record(workload, nq, K, L, achieved_gbs, launch_bytes, kern_ms):
+    rec = measured_traffic(workload, nq, K, L)
+    out = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+           "traffic": rec["traffic_bytes"] if rec else None,
+           "bytes_per_launch": round(launch_bytes), "kernel_ms": round(kern_ms, 4)}
+    if rec:
+        out["traffic_note"] = (f"rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
+                               f"profiles/search_kernel_{workload}_counters.md; L2 hit rate {rec['l2_hit_rate']}")
+    return out
 
 
 def serve_sweep(ix, queries, gt10, w, rates, seconds):
