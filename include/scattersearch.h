@@ -62,7 +62,13 @@ typedef struct ss_search_params {
                                  together (ModifiedDeltaStepping,
                                  src/annsearch.cpp:1586-1731): fewer dependent
                                  steps, recall instead of exact parity */
-    uint32_t reserved[2];
+    uint32_t waves_per_query; /* 1 (default): one wavefront per query; 2..8:
+                                 ScatterSearch (MultiThreadSearchArraySimulation-
+                                 WithET, src/annsearch.cpp:839-1025): that many
+                                 wavefronts per query sharing the visited set,
+                                 election at hop 50, stealing; recall instead of
+                                 exact parity; expand_width must be 1 */
+    uint32_t reserved;
 } ss_search_params;
 
 /* Per-query counters written by every search. */

@@ -34,7 +34,8 @@ class SearchParams(ctypes.Structure):
         ("blocks", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
         ("expand_width", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("waves_per_query", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -89,7 +89,7 @@ struct ss_index {
     // Launch parameters chosen by ss_autotune, per pool-size bucket
     // (next power of two of L); dim and degree are fixed per index.
     struct Tuned {
-        uint32_t l_bucket, expand_width, waves_per_block, visited_slots;
+        uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query;
     };
     std::vector<Tuned> tuned;
     // hipGraph cache of the last batched launch
@@ -435,9 +435,13 @@ static ss_search_params resolve_params(const ss_index *ix, uint32_t l_max, const
             r.waves_per_block = t.waves_per_block;
         if (!r.visited_slots)
             r.visited_slots = t.visited_slots;
+        if (!r.waves_per_query)
+            r.waves_per_query = t.waves_per_query;
     }
     if (!r.expand_width)
         r.expand_width = 1;
+    if (!r.waves_per_query)
+        r.waves_per_query = 1;
     return r;
 }
 
@@ -454,8 +458,28 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         set_error("visited_slots must be a power of two in 256..32768");
         return SS_EINVAL;
     }
-    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots, ew);
+    const uint32_t wq = p && p->waves_per_query ? p->waves_per_query : 1;
+    if (wq < 1 || wq > ss::kMaxWavesPerQueryHost || (wq > 1 && ew != 1)) {
+        set_error("waves_per_query must be 1..8, and expand_width 1 when it is above 1");
+        return SS_EINVAL;
+    }
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
+    if (wq > 1) {  // blocks of wq wavefronts, one query per block
+        const size_t per_block = ss::scatter_lds_bytes_per_block(ix->dim_pad, l_max, slots, wq);
+        const uint32_t blocks_cu = (uint32_t)std::min<size_t>(lds_cu / per_block, 32 / wq);
+        if (blocks_cu == 0) {
+            set_error("search state does not fit in LDS: lower L, visited_slots or waves_per_query");
+            return SS_EINVAL;
+        }
+        uint32_t blocks = p && p->blocks ? p->blocks : (uint32_t)ix->num_cus * blocks_cu;
+        blocks = std::max<uint32_t>(1, std::min(blocks, nq));
+        s->wpb = wq;
+        s->slots = slots;
+        s->blocks = blocks;
+        s->lds_wave = per_block / wq;
+        return SS_OK;
+    }
+    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots, ew);
     const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
     if (waves_cu == 0) {
         set_error("search state does not fit in LDS: lower L or visited_slots");
@@ -583,13 +607,19 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.path = d_path;
     a.path_cap = d_path ? path_cap : 0;
     a.expand_width = p && p->expand_width ? p->expand_width : 1;
+    a.waves_per_query = p && p->waves_per_query ? p->waves_per_query : 1;
+    a.num = ix->num;
+    if (a.waves_per_query > 1 && d_path) {
+        set_error("search: expansion paths are recorded by the one-wavefront search only");
+        return SS_EINVAL;
+    }
     hipError_t e;
     if (mode == SS_MODE_PER_QUERY) {
         a.queue = nullptr;
         a.nq = 1;
         for (uint32_t q = 0; q < nq; q++) {
             a.q_first = q;
-            if ((e = ss::launch_search(ix->metric, a, 1, 1, st)) != hipSuccess)
+            if ((e = ss::launch_search(ix->metric, a, 1, sh.wpb, st)) != hipSuccess)
                 return hip_fail(e, "search launch");
         }
         return SS_OK;
@@ -932,55 +962,66 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
         float ms, rec;
     };
     std::vector<Cand> done;
-    const uint32_t widths[3] = {1, 2, 4};
+    // candidates: exact one-wavefront search over waves per block and visited
+    // slots; with a recall allowance also wider expansion and several
+    // wavefronts per query
+    std::vector<ss_search_params> cands;
     const uint32_t wpbs[4] = {1, 2, 4, 8};
     const uint32_t slotss[4] = {1024, 2048, 4096, 8192};
-    float exact_recall = -1.0f;
-    for (uint32_t ew : widths) {
+    for (uint32_t ew : {1u, 2u, 4u}) {
         if (ew > 1 && !(max_recall_loss > 0.0f))
             continue;
-        for (uint32_t slots : slotss) {
+        for (uint32_t slots : slotss)
             for (uint32_t wpb : wpbs) {
                 ss_search_params p{};
                 p.expand_width = ew;
                 p.waves_per_block = wpb;
                 p.visited_slots = slots;
-                Shape sh;
-                if (choose_shape(ix, L, nq, &p, &sh) != SS_OK)
-                    continue;
-                if ((rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr,
-                                     ix->stream)))
-                    break;
-                float total = 0.0f;
-                for (int rep = 0; rep < 3 && !rc; rep++) {
-                    float ms = 0.0f;
-                    if ((e = hipEventRecord(ev0, ix->stream)) ||
-                        (rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr,
-                                         ix->stream)) ||
-                        (e = hipEventRecord(ev1, ix->stream)) || (e = hipEventSynchronize(ev1)) ||
-                        (e = hipEventElapsedTime(&ms, ev0, ev1))) {
-                        if (!rc)
-                            rc = hip_fail(e, "ss_autotune: timing");
-                        break;
-                    }
-                    total += ms;
-                }
-                if (rc)
-                    break;
-                if ((e = hipMemcpy(got.data(), di.ptr, nk * 4, hipMemcpyDeviceToHost)) != hipSuccess) {
-                    rc = hip_fail(e, "ss_autotune: results");
-                    break;
-                }
-                const float rec = recall(got);
-                if (ew == 1)
-                    exact_recall = rec;
-                done.push_back(Cand{p, total / 3.0f, rec});
+                p.waves_per_query = 1;
+                cands.push_back(p);
             }
-            if (rc)
+    }
+    if (max_recall_loss > 0.0f)
+        for (uint32_t wq : {2u, 4u})
+            for (uint32_t slots : slotss) {
+                ss_search_params p{};
+                p.expand_width = 1;
+                p.waves_per_block = wq;
+                p.visited_slots = slots;
+                p.waves_per_query = wq;
+                cands.push_back(p);
+            }
+    float exact_recall = -1.0f;
+    for (const ss_search_params &p : cands) {
+        Shape sh;
+        if (choose_shape(ix, L, nq, &p, &sh) != SS_OK)
+            continue;
+        if ((rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr, ix->stream)))
+            break;
+        float total = 0.0f;
+        for (int rep = 0; rep < 3 && !rc; rep++) {
+            float ms = 0.0f;
+            if ((e = hipEventRecord(ev0, ix->stream)) ||
+                (rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr,
+                                 ix->stream)) ||
+                (e = hipEventRecord(ev1, ix->stream)) || (e = hipEventSynchronize(ev1)) ||
+                (e = hipEventElapsedTime(&ms, ev0, ev1))) {
+                if (!rc)
+                    rc = hip_fail(e, "ss_autotune: timing");
                 break;
+            }
+            total += ms;
         }
         if (rc)
             break;
+        if ((e = hipMemcpy(got.data(), di.ptr, nk * 4, hipMemcpyDeviceToHost)) != hipSuccess) {
+            rc = hip_fail(e, "ss_autotune: results");
+            break;
+        }
+        const float rec = recall(got);
+        if (p.expand_width == 1 && p.waves_per_query == 1)
+            exact_recall = rec;
+        done.push_back(Cand{p, total / 3.0f, rec});
     }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
@@ -998,7 +1039,8 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
     ix->tuned.erase(std::remove_if(ix->tuned.begin(), ix->tuned.end(),
                                    [&](const ss_index::Tuned &t) { return t.l_bucket == bucket; }),
                     ix->tuned.end());
-    ix->tuned.push_back(ss_index::Tuned{bucket, best->p.expand_width, best->p.waves_per_block, best->p.visited_slots});
+    ix->tuned.push_back(ss_index::Tuned{bucket, best->p.expand_width, best->p.waves_per_block, best->p.visited_slots,
+                                        best->p.waves_per_query});
     if (best_out)
         *best_out = best->p;
     if (best_ms_out)

@@ -38,6 +38,8 @@ struct SearchArgs {
     uint32_t *path;       // optional [nq][path_cap] expanded-node ids
     uint32_t path_cap;
     uint32_t expand_width; // 1 (exact), 2 or 4 pool entries expanded per step
+    uint32_t waves_per_query; // 1, or 2..8 wavefronts per query (ScatterSearch, blocks of that many)
+    uint32_t num;          // base points (random entry points of the multi-wavefront search)
 };
 
 // A serving activation: wavefronts claim published requests from a ring in
@@ -84,6 +86,10 @@ struct BruteArgs {
 // Per-wave LDS bytes for a search with the given pool bound and hash size.
 size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
                                  uint32_t expand_width = 1);
+// LDS bytes of one block (one query) of the multi-wavefront search.
+size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+                                   uint32_t waves_per_query);
+constexpr uint32_t kMaxWavesPerQueryHost = 8;
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,

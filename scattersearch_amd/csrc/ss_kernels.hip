@@ -339,17 +339,13 @@ struct QueryState {
     float *out_d;
 };
 
-// Validates K and L, loads the query into LDS, clears the visited hash and
-// seeds the pool with the first min(L, deg(ep)) neighbours of the entry
-// point, marked visited (ep itself is neither pooled nor marked),
-// src/annsearch.cpp:185-215.  Returns false (after writing an empty result)
-// for a request the host should have rejected.
-template <int NBLK, int TAILI, int METRIC, bool RING = false>
-__device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st)
+// Validates K and L, loads the query into LDS and clears the visited hash.
+// Returns false (after writing an empty result) for a request the host
+// should have rejected.
+template <bool RING = false>
+__device__ __forceinline__ bool query_prologue(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st)
 {
     const uint32_t lane = lane_id();
-    const int nblk_rt = (int)(a.dim_pad / 16);
-    const int tail_rt = (int)((a.dim_pad % 16) != 0);
     st.t_start = (uint64_t)wall_clock64();
     st.K = uniform(a.Ks ? ld_word<RING>(a.Ks + q) : a.k_default);
     st.L = uniform(a.Ls ? ld_word<RING>(a.Ls + q) : a.l_default);
@@ -357,6 +353,8 @@ __device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &
     st.out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
     st.lossy = 0;
     st.hops = 0;
+    st.ndc = 0;
+    st.t = 0;
     if (st.K == 0 || st.L == 0 || st.K > st.L || st.L > a.l_max || st.K > a.k_stride) {
         for (uint32_t i = lane; i < a.k_stride; i += 64) {
             st_word<RING>(st.out_ids + i, kNone);
@@ -383,8 +381,20 @@ __device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &
     uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
     for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
         h4[i] = make_uint4(kNone, kNone, kNone, kNone);
-    wave_sync();
-    const uint32_t nb = lane < a.r_pad ? a.adj[(size_t)a.ep * a.r_pad + lane] : kNone;
+    return true;
+}
+
+// Seeds the pool with the first min(L, deg(node)) neighbours of `node`,
+// marked visited and sorted by distance (ties in row order); `node` itself
+// is neither pooled nor marked, src/annsearch.cpp:185-215.  Sets st.t and
+// adds the distance computations to st.ndc.
+template <int NBLK, int TAILI, int METRIC>
+__device__ __forceinline__ void seed_pool(const SearchArgs &a, const WaveLds &w, uint32_t node, QueryState &st)
+{
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    const uint32_t nb = lane < a.r_pad ? a.adj[(size_t)node * a.r_pad + lane] : kNone;
     const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
     const uint32_t t = deg < st.L ? deg : st.L;
     if (lane < t) {
@@ -406,8 +416,18 @@ __device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &
         w.pool_d[rank] = di;
     }
     st.t = t;
-    st.ndc = t;
+    st.ndc += t;
     wave_sync();
+}
+
+// query_prologue + seed_pool from the index's entry point.
+template <int NBLK, int TAILI, int METRIC, bool RING = false>
+__device__ __forceinline__ bool query_begin(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st)
+{
+    if (!query_prologue<RING>(a, w, q, st))
+        return false;
+    wave_sync();
+    seed_pool<NBLK, TAILI, METRIC>(a, w, a.ep, st);
     return true;
 }
 
@@ -436,23 +456,19 @@ __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w,
     wave_sync();
 }
 
-// One query: src/annsearch.cpp:182-259 as a sequence of whole-row
-// expansions.  For the expansion of pool entry k with pool P (size t, tail
-// distance tau = P[t-1].dist), let A be the unvisited neighbours with
-// distance < tau in row order.  The reference's per-neighbour loop (tail
-// filter :238, InsertIntoPool :241 placing an entry after equal distances,
-// t grows to L :242-243) leaves exactly the first min(L, t+|A|) entries of
-// the stable merge of P and A (P first on equal distances, A in row order),
-// and its jump target nk (:244-252) is the merged position of the first
-// smallest element of A.  Every entry before the cursor is explored, so the
-// next node is the first unexplored entry of the merged pool.
-template <int NBLK, int TAILI, int METRIC, int MC, bool RING = false>
-__device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
+// Hook of expand_loop that never stops the search.
+struct NoHook {
+    __device__ __forceinline__ bool operator()(const QueryState &, uint32_t) const { return true; }
+};
+
+// The expansion loop of one wavefront over its pool (see search_one): runs
+// until no unexplored entry is left or hook(st, t) returns false at the
+// start of a step.
+template <int NBLK, int TAILI, int METRIC, int MC, typename Hook>
+__device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st,
+                                            Hook &&hook)
 {
     constexpr int kMaxChunks = MC;  // pool chunks of 64 held in registers (L <= 64 * MC)
-    QueryState st;
-    if (!query_begin<NBLK, TAILI, METRIC, RING>(a, w, q, st))
-        return;
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
@@ -465,13 +481,24 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
     // then, so the load overlaps the merge.
     uint32_t t = st.t;
     uint32_t k = 0;
-    bool have = t > 0;
+    bool have = false;
+    for (uint32_t c0 = 0; c0 < t; c0 += 64) {  // first unexplored entry
+        const uint32_t i = c0 + lane;
+        const uint64_t m = ballot(i < t && !(w.pool_id[i] & kExplored));
+        if (m) {
+            k = c0 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            have = true;
+            break;
+        }
+    }
     uint32_t nb = kNone;
     if (have) {
-        const uint32_t n0 = uniform(w.pool_id[0]);
+        const uint32_t n0 = uniform(w.pool_id[k]);
         nb = lane < a.r_pad ? a.adj[(size_t)n0 * a.r_pad + lane] : kNone;
     }
     while (have) {
+        if (!hook(st, t))
+            break;
         const uint32_t n = uniform(w.pool_id[k]);
         const float tail = readlane_f(w.pool_d[t - 1], 0);
         wave_sync();
@@ -644,7 +671,25 @@ __device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
         wave_sync();
     }
     st.t = t;
+}
 
+// One query: src/annsearch.cpp:182-259 as a sequence of whole-row
+// expansions.  For the expansion of pool entry k with pool P (size t, tail
+// distance tau = P[t-1].dist), let A be the unvisited neighbours with
+// distance < tau in row order.  The reference's per-neighbour loop (tail
+// filter :238, InsertIntoPool :241 placing an entry after equal distances,
+// t grows to L :242-243) leaves exactly the first min(L, t+|A|) entries of
+// the stable merge of P and A (P first on equal distances, A in row order),
+// and its jump target nk (:244-252) is the merged position of the first
+// smallest element of A.  Every entry before the cursor is explored, so the
+// next node is the first unexplored entry of the merged pool.
+template <int NBLK, int TAILI, int METRIC, int MC, bool RING = false>
+__device__ void search_one(const SearchArgs &a, const WaveLds &w, uint32_t q)
+{
+    QueryState st;
+    if (!query_begin<NBLK, TAILI, METRIC, RING>(a, w, q, st))
+        return;
+    expand_loop<NBLK, TAILI, METRIC, MC>(a, w, q, st, NoHook{});
     query_end<RING>(a, w, q, st);
 }
 
@@ -871,6 +916,486 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
 }
 
 __global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
+
+// ---------------------------------------------------------------------------
+// Several wavefronts per query: ScatterSearch
+// (MultiThreadSearchArraySimulationWithET, src/annsearch.cpp:839-1025, and
+// SearchUntilBestThreadStop, :1368-1455) with the W wavefronts of a block in
+// place of the reference's W OpenMP threads.  They share the query and the
+// visited hash in LDS; each has its own pool.  Wavefront 0 starts from the
+// index's entry point, the others from pseudo-random nodes (the reference
+// draws every thread's start with rand(), :876).  Each runs the exact
+// expansion loop; at its 50th expansion it enters the election with its
+// K-th distance (:911-920); once all have, a wavefront whose best entry is
+// worse than the elected K-th distance leaves its loop (:921-928).  The
+// elected wavefront ends the query for everyone when its own loop ends
+// (:967-970); the others, when their loop ends first, take the unexplored
+// entries of a good wavefront that passed the election (marking them
+// explored there) and continue (:1374-1454).  The result is the elected
+// pool's top K merged with the others' (:990-999).  Like the reference the
+// outcome depends on timing; it is judged by recall.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxWavesPerQuery = kMaxWavesPerQueryHost;
+constexpr uint32_t kElectionHop = 50;  // src/annsearch.cpp:858
+
+struct ScatterCtl {
+    unsigned long long best;  // (order-preserving bits of the K-th distance << 32) | wavefront
+    uint32_t decide;          // wavefronts past the election hop
+    uint32_t finish;          // the elected wavefront has finished
+    int32_t good[kMaxWavesPerQuery];
+    uint32_t reached[kMaxWavesPerQuery];
+    uint32_t t[kMaxWavesPerQuery];
+    uint32_t ndc[kMaxWavesPerQuery];
+    uint32_t hops[kMaxWavesPerQuery];
+    uint32_t lossy[kMaxWavesPerQuery];
+    uint32_t q;
+    uint32_t pad[3];
+};
+
+// Per wavefront: pool, merge staging (both l_max entries) and 64 candidates.
+__host__ __device__ inline size_t scatter_wave_bytes(uint32_t l_max)
+{
+    return align16(l_max * 4u) * 4 + 64 * 4 * 2;
+}
+
+__host__ __device__ inline size_t scatter_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+                                                    uint32_t W)
+{
+    return align16(dim_pad * 4u) + align16(hash_slots * 4u) + align16(sizeof(ScatterCtl)) +
+           (size_t)W * scatter_wave_bytes(l_max);
+}
+
+struct ScatterLds {
+    WaveLds w;  // this wavefront's view: shared query and hash, own pool and candidates
+    uint32_t *tmp_id;
+    float *tmp_d;
+    ScatterCtl *ctl;
+    unsigned char *waves;  // start of the per-wavefront areas
+};
+
+__device__ inline ScatterLds scatter_carve(unsigned char *p, uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+                                           uint32_t wave)
+{
+    ScatterLds s;
+    s.w.qry = reinterpret_cast<float *>(p);
+    p += align16(dim_pad * 4u);
+    s.w.hash = reinterpret_cast<uint32_t *>(p);
+    p += align16(hash_slots * 4u);
+    s.ctl = reinterpret_cast<ScatterCtl *>(p);
+    p += align16(sizeof(ScatterCtl));
+    s.waves = p;
+    p += (size_t)wave * scatter_wave_bytes(l_max);
+    s.w.pool_id = reinterpret_cast<uint32_t *>(p);
+    p += align16(l_max * 4u);
+    s.w.pool_d = reinterpret_cast<float *>(p);
+    p += align16(l_max * 4u);
+    s.tmp_id = reinterpret_cast<uint32_t *>(p);
+    p += align16(l_max * 4u);
+    s.tmp_d = reinterpret_cast<float *>(p);
+    p += align16(l_max * 4u);
+    s.w.cand_id = reinterpret_cast<uint32_t *>(p);
+    p += 64 * 4;
+    s.w.cand_d = reinterpret_cast<float *>(p);
+    return s;
+}
+
+__device__ __forceinline__ uint32_t *wave_pool_id(const ScatterLds &s, uint32_t l_max, uint32_t wave)
+{
+    return reinterpret_cast<uint32_t *>(s.waves + (size_t)wave * scatter_wave_bytes(l_max));
+}
+
+__device__ __forceinline__ float *wave_pool_d(const ScatterLds &s, uint32_t l_max, uint32_t wave)
+{
+    return reinterpret_cast<float *>(s.waves + (size_t)wave * scatter_wave_bytes(l_max) + align16(l_max * 4u));
+}
+
+__device__ __forceinline__ uint32_t *wave_tmp_id(const ScatterLds &s, uint32_t l_max, uint32_t wave)
+{
+    return reinterpret_cast<uint32_t *>(s.waves + (size_t)wave * scatter_wave_bytes(l_max) + align16(l_max * 4u) * 2);
+}
+
+__device__ __forceinline__ float *wave_tmp_d(const ScatterLds &s, uint32_t l_max, uint32_t wave)
+{
+    return reinterpret_cast<float *>(s.waves + (size_t)wave * scatter_wave_bytes(l_max) + align16(l_max * 4u) * 3);
+}
+
+__device__ __forceinline__ uint32_t order_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float order_float(uint32_t o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T lds_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The per-step checks of src/annsearch.cpp:908-928 for one wavefront.
+struct ElectionHook {
+    ScatterCtl *ctl;
+    const WaveLds *w;
+    uint32_t wave, W;
+    bool decided, identify;
+
+    __device__ __forceinline__ bool operator()(QueryState &st, uint32_t t)
+    {
+        if (uniform(lds_load(&ctl->finish)))
+            return false;
+        if (!decided && st.hops == kElectionHop) {
+            decided = true;
+            if (lane_id() == 0) {
+                const float kth = st.K <= t ? w->pool_d[st.K - 1] : __builtin_inff();
+                ctl->reached[wave] = 1u;
+                atomicMin(&ctl->best, ((unsigned long long)order_bits(kth) << 32) | wave);
+                atomicAdd(&ctl->decide, 1u);
+            }
+        }
+        if (identify && uniform(lds_load(&ctl->decide)) == W) {
+            identify = false;
+            const float best = order_float((uint32_t)(lds_load(&ctl->best) >> 32));
+            const float p0 = readlane_f(w->pool_d[0], 0);
+            if (best < p0) {
+                if (lane_id() == 0)
+                    ctl->good[wave] = -1;
+                return false;
+            }
+            if (best > p0 && lane_id() == 0)
+                ctl->good[wave] = 1;
+        }
+        return true;
+    }
+};
+
+struct UntilFinishHook {
+    ScatterCtl *ctl;
+    __device__ __forceinline__ bool operator()(QueryState &, uint32_t) const
+    {
+        return !uniform(lds_load(&ctl->finish));
+    }
+};
+
+// Merges sorted list B (nb entries) into sorted list A (na entries), keeping
+// the first `keep` in out: A first on equal distances, entries of B whose id
+// is already in A dropped (InsertIntoPool's duplicate rule, include/util.h:
+// 65-74).  B is compacted in place.  out may not alias A or B.  Returns the
+// new size.
+__device__ uint32_t merge_sorted(const uint32_t *a_id, const float *a_d, uint32_t na, uint32_t *b_id, float *b_d,
+                                 uint32_t nb, uint32_t keep, uint32_t *out_id, float *out_d)
+{
+    const uint32_t lane = lane_id();
+    uint32_t nbk = 0;
+    for (uint32_t j0 = 0; j0 < nb; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        bool keep_j = false;
+        uint32_t id = kNone;
+        float d = 0.f;
+        if (j < nb) {
+            id = b_id[j];
+            d = b_d[j];
+            uint32_t lo = 0, hi = na;  // first A entry with distance > d
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a_d[mid] <= d)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            keep_j = true;
+            for (uint32_t i = lo; i-- > 0;) {
+                if (a_d[i] < d)
+                    break;
+                if ((a_id[i] & ~kExplored) == (id & ~kExplored)) {
+                    keep_j = false;
+                    break;
+                }
+            }
+        }
+        const uint64_t m = ballot(keep_j);
+        wave_sync();
+        if (keep_j) {
+            b_id[nbk + lanes_below(m)] = id;
+            b_d[nbk + lanes_below(m)] = d;
+        }
+        nbk += (uint32_t)__popcll(m);
+        wave_sync();
+    }
+    for (uint32_t i0 = 0; i0 < na; i0 += 64) {  // A[i] goes to i + #{b < a}
+        const uint32_t i = i0 + lane;
+        if (i < na) {
+            const float d = a_d[i];
+            uint32_t lo = 0, hi = nbk;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (b_d[mid] < d)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (i + lo < keep) {
+                out_id[i + lo] = a_id[i];
+                out_d[i + lo] = d;
+            }
+        }
+    }
+    for (uint32_t j0 = 0; j0 < nbk; j0 += 64) {  // B[j] goes to j + #{a <= b}
+        const uint32_t j = j0 + lane;
+        if (j < nbk) {
+            const float d = b_d[j];
+            uint32_t lo = 0, hi = na;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a_d[mid] <= d)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (j + lo < keep) {
+                out_id[j + lo] = b_id[j];
+                out_d[j + lo] = d;
+            }
+        }
+    }
+    wave_sync();
+    const uint32_t n = na + nbk;
+    return n < keep ? n : keep;
+}
+
+__device__ __forceinline__ void copy_list(const uint32_t *src_id, const float *src_d, uint32_t n, uint32_t *dst_id,
+                                          float *dst_d)
+{
+    for (uint32_t i = lane_id(); i < n; i += 64) {
+        dst_id[i] = src_id[i];
+        dst_d[i] = src_d[i];
+    }
+    wave_sync();
+}
+
+// SearchUntilBestThreadStop's stealing step (src/annsearch.cpp:1374-1399):
+// up to ten random picks of a good wavefront past the election; its
+// unexplored entries 1..L-1 are claimed (marked explored there) and merged
+// into this wavefront's pool without growing it.  The claimed ids get their
+// distances recomputed, since the owner may be moving its entries while they
+// are read.
+template <int NBLK, int TAILI, int METRIC>
+__device__ void steal_entries(const SearchArgs &a, const ScatterLds &s, uint32_t wave, uint32_t W, uint32_t q,
+                              QueryState &st)
+{
+    const WaveLds &w = s.w;
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    int target = -1;
+    for (uint32_t round = 0; round < 10; round++) {
+        if (uniform(lds_load(&s.ctl->finish)))
+            break;
+        const uint32_t i = mix32(q * 0x9E3779B1u + wave * 0x632BE5ABu + round * 0x85EBCA6Bu + 0x27D4EB2Fu) % W;
+        if (i != wave && uniform(lds_load(&s.ctl->reached[i])) &&
+            (int)uniform((uint32_t)lds_load(&s.ctl->good[i])) == 1) {
+            target = (int)i;
+            break;
+        }
+    }
+    if (target < 0 || st.t == 0)
+        return;
+    uint32_t *vic = wave_pool_id(s, a.l_max, (uint32_t)target);
+    for (uint32_t i0 = 1; i0 < st.L; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        uint32_t id = kNone;
+        if (i < st.L) {
+            const uint32_t old = atomicOr(&vic[i], kExplored);
+            if (!(old & kExplored))
+                id = old;
+        }
+        const uint64_t m = ballot(id != kNone);
+        const uint32_t nc = (uint32_t)__popcll(m);
+        if (nc == 0)
+            continue;
+        if (id != kNone)
+            w.cand_id[lanes_below(m)] = id;
+        wave_sync();
+        candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
+        wave_sync();
+        st.ndc += nc;
+        // sort the batch by (distance, claim order)
+        const float cd = lane < nc ? w.cand_d[lane] : __builtin_inff();
+        const uint32_t cid = lane < nc ? w.cand_id[lane] : kNone;
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < nc; j++) {
+            const float dj = readlane_f(cd, j);
+            rank += (dj < cd || (dj == cd && j < lane)) ? 1u : 0u;
+        }
+        wave_sync();
+        if (lane < nc) {
+            w.cand_id[rank] = cid;
+            w.cand_d[rank] = cd;
+        }
+        wave_sync();
+        const uint32_t t = merge_sorted(w.pool_id, w.pool_d, st.t, w.cand_id, w.cand_d, nc, st.t, s.tmp_id, s.tmp_d);
+        copy_list(s.tmp_id, s.tmp_d, t, w.pool_id, w.pool_d);
+    }
+}
+
+// One query with W wavefronts (the block); every wavefront of the block
+// calls this with the same q.
+template <int NBLK, int TAILI, int METRIC, int MC>
+__device__ void scatter_query(const SearchArgs &a, const ScatterLds &s, uint32_t wave, uint32_t W, uint32_t q)
+{
+    const WaveLds &w = s.w;
+    const uint32_t lane = lane_id();
+    QueryState st;
+    // wavefront 0 validates, loads the query and clears the shared hash and
+    // the control block; the others read K and L themselves
+    bool ok;
+    if (wave == 0) {
+        ok = query_prologue<false>(a, w, q, st);
+        if (lane == 0) {
+            ScatterCtl *c = s.ctl;
+            c->best = ~0ull;
+            c->decide = 0;
+            c->finish = 0;
+            for (uint32_t i = 0; i < kMaxWavesPerQuery; i++) {
+                c->good[i] = 0;
+                c->reached[i] = 0;
+            }
+        }
+    } else {
+        st.t_start = (uint64_t)wall_clock64();
+        st.K = uniform(a.Ks ? a.Ks[q] : a.k_default);
+        st.L = uniform(a.Ls ? a.Ls[q] : a.l_default);
+        st.lossy = st.hops = st.ndc = st.t = 0;
+        st.out_ids = a.out_ids + (size_t)q * a.k_stride;
+        st.out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
+        ok = !(st.K == 0 || st.L == 0 || st.K > st.L || st.L > a.l_max || st.K > a.k_stride);
+    }
+    __syncthreads();
+    if (ok) {
+        const uint32_t ep = wave == 0 ? a.ep : mix32(q * 0x9E3779B1u ^ (wave * 0x85EBCA6Bu) ^ 0x5BD1E995u) % a.num;
+        seed_pool<NBLK, TAILI, METRIC>(a, w, ep, st);
+        for (uint32_t i = st.t + lane; i < st.L; i += 64)
+            w.pool_id[i] = kNone;  // explored and empty: never stolen
+        wave_sync();
+        ElectionHook eh{s.ctl, &w, wave, W, false, true};
+        expand_loop<NBLK, TAILI, METRIC, MC>(a, w, q, st, eh);
+        const unsigned long long best = lds_load(&s.ctl->best);
+        if (best != ~0ull && (uint32_t)(best & 0xFFu) == wave) {
+            if (lane == 0)
+                __hip_atomic_store(&s.ctl->finish, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (!uniform(lds_load(&s.ctl->finish))) {
+            steal_entries<NBLK, TAILI, METRIC>(a, s, wave, W, q, st);
+            expand_loop<NBLK, TAILI, METRIC, MC>(a, w, q, st, UntilFinishHook{s.ctl});
+        }
+        if (lane == 0) {
+            s.ctl->t[wave] = st.t;
+            s.ctl->ndc[wave] = st.ndc;
+            s.ctl->hops[wave] = st.hops;
+        }
+        uint32_t lsum = st.lossy;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            lsum += __shfl_xor(lsum, off, 64);
+        if (lane == 0)
+            s.ctl->lossy[wave] = lsum;
+    }
+    __syncthreads();
+    if (!ok || wave != 0)
+        return;
+    // wavefront 0: the elected pool's top K merged with every other pool's
+    // (src/annsearch.cpp:990-999); if no wavefront reached the election,
+    // the pool with the smallest K-th distance leads
+    const unsigned long long bk = lds_load(&s.ctl->best);
+    uint32_t lead = 0;
+    if (bk != ~0ull) {
+        lead = (uint32_t)(bk & 0xFFu);
+    } else {
+        float bd = __builtin_inff();
+        for (uint32_t i = 0; i < W; i++) {
+            const uint32_t ti = s.ctl->t[i];
+            const float d = ti >= st.K ? wave_pool_d(s, a.l_max, i)[st.K - 1] : __builtin_inff();
+            if (d < bd) {
+                bd = d;
+                lead = i;
+            }
+        }
+    }
+    const uint32_t K = st.K;
+    // the result alternates between two staging areas (wavefront 0's and the
+    // leader's, or wavefront 1's when wavefront 0 leads); the other pools are
+    // merged in wavefront order and compacted in place
+    uint32_t *r_id = wave_tmp_id(s, a.l_max, 0), *o_id = wave_tmp_id(s, a.l_max, lead != 0 ? lead : 1);
+    float *r_d = wave_tmp_d(s, a.l_max, 0), *o_d = wave_tmp_d(s, a.l_max, lead != 0 ? lead : 1);
+    uint32_t nr = s.ctl->t[lead] < K ? s.ctl->t[lead] : K;
+    copy_list(wave_pool_id(s, a.l_max, lead), wave_pool_d(s, a.l_max, lead), nr, r_id, r_d);
+    for (uint32_t i = 0; i < W; i++) {
+        if (i == lead)
+            continue;
+        const uint32_t ti = s.ctl->t[i] < K ? s.ctl->t[i] : K;
+        nr = merge_sorted(r_id, r_d, nr, wave_pool_id(s, a.l_max, i), wave_pool_d(s, a.l_max, i), ti, K, o_id,
+                          o_d);
+        uint32_t *xi = r_id;
+        float *xd = r_d;
+        r_id = o_id;
+        r_d = o_d;
+        o_id = xi;
+        o_d = xd;
+    }
+    uint32_t ndc = 0, hops = 0, lossy = 0;
+    for (uint32_t i = 0; i < W; i++) {
+        ndc += s.ctl->ndc[i];
+        hops += s.ctl->hops[i];
+        lossy += s.ctl->lossy[i];
+    }
+    for (uint32_t i = lane; i < a.k_stride; i += 64) {
+        const bool has = i < K && i < nr;
+        st.out_ids[i] = has ? (r_id[i] & ~kExplored) : kNone;
+        if (st.out_d)
+            st.out_d[i] = has ? r_d[i] : __builtin_inff();
+    }
+    if (a.stats && lane == 0)
+        a.stats[q] = ss_query_stats{ndc, hops, lossy, 0u, st.t_start, (uint64_t)wall_clock64()};
+}
+
+// Blocks of W wavefronts, one query per block at a time.
+template <int NBLK, int TAILI, int METRIC, int MC>
+__global__ __launch_bounds__(512, search_min_waves<NBLK>()) void scatter_kernel(SearchArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t W = blockDim.x >> 6;
+    const ScatterLds s = scatter_carve(smem, a.dim_pad, a.l_max, a.hash_slots, wave);
+    bool first = true;
+    while (true) {
+        if (a.queue) {
+            if (threadIdx.x == 0)
+                s.ctl->q = atomicAdd(a.queue, 1u);
+        } else if (threadIdx.x == 0) {
+            s.ctl->q = first ? blockIdx.x : 0xFFFFFFFFu;
+        }
+        first = false;
+        __syncthreads();
+        const uint32_t q = uniform(s.ctl->q);
+        __syncthreads();
+        if (q >= a.nq)
+            break;
+        scatter_query<NBLK, TAILI, METRIC, MC>(a, s, wave, W, a.q_first + q);
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Serving activation.  Requests are published by the host (ring slots filled,
@@ -1226,8 +1751,25 @@ struct SearchLaunch {
         hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC, EW, MC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
         return hipGetLastError();
     }
+    template <int MC>
+    static hipError_t go_scatter(const SearchArgs &a, uint32_t blocks, hipStream_t st)
+    {
+        const uint32_t W = a.waves_per_query;
+        const size_t lds = scatter_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, W);
+        hipLaunchKernelGGL((scatter_kernel<NBLK, TAILI, METRIC, MC>), dim3(blocks), dim3(64 * W), lds, st, a);
+        return hipGetLastError();
+    }
     static hipError_t run(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
+        if (a.waves_per_query > 1) {
+            if (a.l_max <= 64)
+                return go_scatter<1>(a, blocks, st);
+            if (a.l_max <= 128)
+                return go_scatter<2>(a, blocks, st);
+            if (a.l_max <= 256)
+                return go_scatter<4>(a, blocks, st);
+            return go_scatter<kPoolChunks>(a, blocks, st);
+        }
         switch (a.expand_width) {
         case 2:
             return go<2, kPoolChunks>(a, blocks, wpb, st);
@@ -1250,9 +1792,18 @@ struct SearchLaunch {
         return hipFuncSetAttribute(reinterpret_cast<const void *>(search_kernel<NBLK, TAILI, METRIC, EW, MC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
     }
+    template <int MC>
+    static hipError_t attr_scatter(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(scatter_kernel<NBLK, TAILI, METRIC, MC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
     static hipError_t attr(size_t max_lds)
     {
         hipError_t e;
+        if ((e = attr_scatter<1>(max_lds)) != hipSuccess || (e = attr_scatter<2>(max_lds)) != hipSuccess ||
+            (e = attr_scatter<4>(max_lds)) != hipSuccess || (e = attr_scatter<kPoolChunks>(max_lds)) != hipSuccess)
+            return e;
         if ((e = attr1<1, 1>(max_lds)) != hipSuccess || (e = attr1<1, 2>(max_lds)) != hipSuccess ||
             (e = attr1<1, 4>(max_lds)) != hipSuccess || (e = attr1<1, kPoolChunks>(max_lds)) != hipSuccess ||
             (e = attr1<2, kPoolChunks>(max_lds)) != hipSuccess)
@@ -1331,6 +1882,11 @@ hipError_t attr_all(size_t max_lds)
 size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots, uint32_t expand_width)
 {
     return wave_lds_bytes(dim_pad, l_max, hash_slots, expand_width);
+}
+
+size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots, uint32_t waves_per_query)
+{
+    return scatter_lds_bytes(dim_pad, l_max, hash_slots, waves_per_query);
 }
 
 size_t bruteforce_lds_bytes(uint32_t dim_pad) { return bf_lds_bytes(dim_pad); }

@@ -69,3 +69,17 @@ def test_d96_persistent_graph_batches(d96, batch):
         assert np.array_equal(ids, o[0])
         assert np.array_equal(ds.view(np.uint32), o[1].view(np.uint32))
         assert np.array_equal(st["hops"], o[3])
+
+
+def test_ip_d200_scatter_recall(ip200):
+    """Several wavefronts per query on the inner-product workload, judged by
+    recall@10 against exact ground truth."""
+    from scattersearch_amd import Index, metrics
+    d = ip200
+    ix = Index(d["base"], 1)
+    ix.set_graph(d["adj"], d["ep"])
+    gt, _ = ix.bruteforce(d["queries"], 10)
+    exact = ix.search(d["queries"], 10, 128, visited_slots=16384)[0]
+    ids, ds, st = ix.search(d["queries"], 10, 128, waves_per_query=4, visited_slots=16384)
+    assert metrics.recall(ids, gt, 10) >= metrics.recall(exact, gt, 10) - 0.03
+    assert (np.diff(ds[:, :10], axis=1) >= 0).all()

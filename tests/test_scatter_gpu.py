@@ -1,0 +1,90 @@
+"""Several wavefronts per query: ScatterSearch
+(MultiThreadSearchArraySimulationWithET and SearchUntilBestThreadStop,
+src/annsearch.cpp:839-1025 and 1368-1455).  Like the reference, the outcome
+depends on timing (election, early exits, stealing), so it is judged by
+recall against exact ground truth; every returned entry must still be a real
+base point at its exact distance, in order, without repeats.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(d, metric=0):
+    from scattersearch_amd import Index
+    ix = Index(d["base"], metric)
+    ix.set_graph(d["adj"], d["ep"])
+    return ix
+
+
+def _check_entries(base, q, ids, ds, K, metric=0):
+    import oracle
+    for i in range(q.shape[0]):
+        k = int(K) if np.ndim(K) == 0 else int(K[i])
+        row, dr = ids[i, :k], ds[i, :k]
+        assert (row != 0xFFFFFFFF).all(), i
+        assert len(set(row.tolist())) == k, i
+        assert (np.diff(dr) >= 0).all(), i
+        for j in range(k):
+            want = np.float32(oracle.dist(metric, base[row[j]], q[i]))
+            assert want.view(np.uint32) == dr[j].view(np.uint32), (i, j)
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+
+
+@pytest.mark.parametrize("W", [2, 4, 8])
+def test_scatter_recall(c0, W):
+    import oracle
+    from scattersearch_amd import metrics
+    gt, _ = oracle.bruteforce(c0["base"], c0["queries"], 10)
+    ix = _index(c0)
+    exact = ix.search(c0["queries"], 10, 64)
+    ids, ds, st = ix.search(c0["queries"], 10, 64, waves_per_query=W, visited_slots=8192)
+    r1, rw = metrics.recall(exact[0], gt, 10), metrics.recall(ids, gt, 10)
+    assert rw >= r1 - 0.03, (W, r1, rw)
+    _check_entries(c0["base"], c0["queries"], ids, ds, 10)
+    assert (st["ndc"] > 0).all() and (st["hops"] > 0).all()
+
+
+def test_scatter_mixed_k_and_l_one_launch(small):
+    import oracle
+    from scattersearch_amd import metrics
+    rng = np.random.default_rng(9)
+    q = small["queries"]
+    nq = q.shape[0]
+    K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(max(int(k), 16), 129) for k in K], dtype=np.uint32)
+    ix = _index(small)
+    ids, ds, st = ix.search(q, K, L, k_stride=32, waves_per_query=4, visited_slots=8192)
+    _check_entries(small["base"], q, ids, ds, K)
+    gt, _ = oracle.bruteforce(small["base"], q, 10)
+    sel = K == 10
+    exact = ix.search(q[sel], 10, L[sel])[0]
+    assert metrics.recall(ids[sel][:, :10], gt[sel], 10) >= metrics.recall(exact, gt[sel], 10) - 0.05
+
+
+def test_scatter_repeated_launches_stay_valid(small):
+    ix = _index(small)
+    q = small["queries"]
+    for rep in range(4):
+        ids, ds, st = ix.search(q, 10, 48, waves_per_query=2, visited_slots=4096, use_graph=(rep % 2 == 1))
+        _check_entries(small["base"], q, ids, ds, 10)
+
+
+def test_scatter_per_query_launches(small):
+    from scattersearch_amd import SS_MODE_PER_QUERY
+    ix = _index(small)
+    q = small["queries"][:16]
+    ids, ds, st = ix.search(q, 10, 48, mode=SS_MODE_PER_QUERY, waves_per_query=4, visited_slots=4096)
+    _check_entries(small["base"], q, ids, ds, 10)
+
+
+def test_scatter_rejects_bad_parameters(small):
+    from scattersearch_amd import SSError
+    from scattersearch_amd._lib import SS_EINVAL
+    ix = _index(small)
+    q = small["queries"][:4]
+    for kw in ({"waves_per_query": 9}, {"waves_per_query": 2, "expand_width": 2}):
+        with pytest.raises(SSError) as e:
+            ix.search(q, 10, 48, **kw)
+        assert e.value.code == SS_EINVAL

@@ -266,3 +266,6 @@ def test_autotune_exact_and_relaxed(small):
     relaxed = ix.autotune(small["queries"], 10, 48, max_recall_loss=0.05)
     assert relaxed["recall"] >= r_exact - 0.05 - 1e-6
     assert relaxed["expand_width"] in (1, 2, 4)
+    assert relaxed["waves_per_query"] in (1, 2, 4)
+    r_tuned = metrics.recall(ix.search(small["queries"], 10, 48)[0], gt, 10)  # relaxed parameters apply
+    assert r_tuned >= r_exact - 0.05 - 1e-6
