@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the wider-expansion and several-wavefronts-per-query measurements")
     ap.add_argument("--visited-slots", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--graph-cache", default="",
@@ -115,10 +117,10 @@ def main():
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()  # default priority; hipGraph capture needs a non-null stream
 
-    def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block):
+    def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block, ew=0, wq=0):
         ix.search_device(dq.data_ptr(), nq, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
                          d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=use_graph,
-                         visited_slots=slots, waves_per_block=wpb)
+                         visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq)
 
     def stats_host():
         return d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
@@ -142,16 +144,16 @@ def main():
     gt, _ = ix.bruteforce(queries, K)
     log(f"ground truth in {time.time() - t0:.1f}s")
 
-    def timed(L, steps, warmup):
+    def timed(L, steps, warmup, **kw):
         for _ in range(warmup):
-            launch(L)
+            launch(L, **kw)
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(steps)]
         t_wall = time.perf_counter()
         for a, b in evs:
             a.record(stream)
-            launch(L)
+            launch(L, **kw)
             b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_wall
@@ -232,6 +234,28 @@ def main():
     plain_qps = npq / (time.perf_counter() - t)
     log(f"plain per-query launches: {plain_qps:.0f} QPS over {npq} queries")
 
+    # Relaxed variants, judged by recall: wider expansion (ModifiedDeltaStepping)
+    # and several wavefronts per query (ScatterSearch).
+    variants = []
+    if not args.no_variants:
+        for ew, wq in ((2, 1), (4, 1), (1, 2), (1, 4), (1, 8)):
+            rows = []
+            for Lv in sorted(set([32, 40, 48, 64, L])):
+                slots = 8192 if wq > 1 else 0
+                try:
+                    wall_v, kern_v = timed(Lv, 3, 1, slots=slots, ew=ew, wq=wq)
+                except Exception as exc:  # a shape that does not fit is reported, not fatal
+                    rows.append(dict(L=Lv, error=str(exc)))
+                    continue
+                st_v = stats_host()
+                rows.append(dict(L=Lv, recall=round(metrics.recall(d_ids.cpu().numpy().view(np.uint32), gt, K), 4),
+                                 qps=round(nq * 3 / wall_v, 1), kernel_ms=round(kern_v, 3),
+                                 mean_ndc=round(float(st_v["ndc"].mean()), 1)))
+            ok = [r for r in rows if r.get("recall", 0) >= args.recall_target]
+            best_v = max(ok, key=lambda r: r["qps"]) if ok else None
+            variants.append(dict(expand_width=ew, waves_per_query=wq, best_at_target=best_v, sweep=rows))
+            log(f"variant ew={ew} wq={wq}: best at recall>={args.recall_target}: {best_v}")
+
     serving = None
     if args.serve_rates:
         try:
@@ -265,6 +289,7 @@ def main():
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
         "serving": serving,
+        "variants": variants,
         "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
                        "service_p99": round(metrics.percentile(service_ms, 0.99), 4),
                        "batch_completion_p50": round(metrics.percentile(done_ms, 0.5), 4),

@@ -120,6 +120,26 @@ class ANNSearch {
             indices[i] = Neighbor(ids[i], d[i], false);
     }
 
+    // src/annsearch.cpp:839-1025 (ScatterSearch): num_threads wavefronts (2..8)
+    // search the query together, sharing the visited set; indices holds K
+    // entries.  Judged by recall: the result depends on timing.
+    void MultiThreadSearchArraySimulationWithET(const float *query, unsigned query_id, int K, int L,
+                                                int num_threads, ss_compat::visited_bits &flags,
+                                                std::vector<unsigned> &indices)
+    {
+        (void)query_id;
+        (void)flags;
+        ss_search_params p{};
+        p.waves_per_query = num_threads > 1 ? (uint32_t)num_threads : 1u;
+        ss_query_stats st{};
+        std::vector<float> d(K);
+        indices.resize(K);
+        check(ss_search(index_, query, 1, nullptr, nullptr, (uint32_t)K, (uint32_t)L, (uint32_t)K, &p,
+                        indices.data(), d.data(), &st));
+        dist_comps += st.ndc;
+        hop_count += st.hops;
+    }
+
     // All queries ([nq][dim]) in one persistent launch; K, L per query or
     // uniform (nullptr).  ids/dists are [nq][K_max].
     void SearchBatch(const float *queries, unsigned nq, const uint32_t *K, const uint32_t *L, uint32_t k_uniform,
