@@ -232,6 +232,17 @@ int ss_serve_replay(ss_index *index, const ss_serve_config *config, const float 
 /* Host-side files and graph construction (no GPU needed).                  */
 /* ---------------------------------------------------------------------- */
 
+/* load_fvecs (src/util.cpp:6-38) and its integer / byte siblings: records
+ * [i32 dim][dim elements] (f32 for fvecs, i32 for ivecs, u8 for bvecs).
+ * ss_vecs_header reads num and dim (elem_size 4 or 1); every record's
+ * dimension is checked on reading.  bvecs elements are widened to float. */
+int ss_vecs_header(const char *path, uint32_t elem_size, uint32_t *num, uint32_t *dim);
+int ss_fvecs_read(const char *path, float *data, uint32_t num, uint32_t dim);
+int ss_ivecs_read(const char *path, uint32_t *data, uint32_t num, uint32_t dim);
+int ss_bvecs_read(const char *path, float *data, uint32_t num, uint32_t dim);
+int ss_fvecs_write(const char *path, const float *data, uint32_t num, uint32_t dim);
+int ss_ivecs_write(const char *path, const uint32_t *data, uint32_t num, uint32_t dim);
+
 /* load_fbin (src/util.cpp:40-70): [u32 num][u32 dim][num*dim f32].
  * ss_fbin_header reads the sizes; ss_fbin_read fills data [num][dim]. */
 int ss_fbin_header(const char *path, uint32_t *num, uint32_t *dim);

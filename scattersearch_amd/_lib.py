@@ -131,6 +131,12 @@ _SIGS = {
     "ss_fbin_header": (cint, [cstr, u32p, u32p]),
     "ss_fbin_read": (cint, [cstr, f32p, u32, u32]),
     "ss_fbin_write": (cint, [cstr, f32p, u32, u32]),
+    "ss_vecs_header": (cint, [cstr, u32, u32p, u32p]),
+    "ss_fvecs_read": (cint, [cstr, f32p, u32, u32]),
+    "ss_ivecs_read": (cint, [cstr, u32p, u32, u32]),
+    "ss_bvecs_read": (cint, [cstr, f32p, u32, u32]),
+    "ss_fvecs_write": (cint, [cstr, f32p, u32, u32]),
+    "ss_ivecs_write": (cint, [cstr, u32p, u32, u32]),
     "ss_gt_header": (cint, [cstr, u32p, u32p]),
     "ss_gt_read": (cint, [cstr, u32p, u32, u32]),
     "ss_gt_write": (cint, [cstr, u32p, u32, u32]),

@@ -151,6 +151,79 @@ void mrng_prune(const float *base, uint32_t dim, int metric, float alpha, std::v
 
 }  // namespace
 
+// The "vecs" formats (load_fvecs, src/util.cpp:6-38): every record is
+// [i32 dim][dim elements]; fvecs elements are f32, ivecs i32, bvecs u8.  The
+// reference takes dim from the first record and num = size / (4 + 4 dim)
+// without checking; here every record's dim is checked.
+static int vecs_header(const char *path, uint32_t elem, uint32_t *num, uint32_t *dim)
+{
+    if (elem != 1 && elem != 4) {
+        set_error("vecs: element size must be 1 or 4");
+        return SS_EINVAL;
+    }
+    File F(path, "rb");
+    if (!F.f)
+        return io_fail("cannot open vecs file", path);
+    const long long sz = file_size(F.f);
+    uint32_t d = 0;
+    if (sz == 0) {
+        if (num)
+            *num = 0;
+        if (dim)
+            *dim = 0;
+        return SS_OK;
+    }
+    if (!read_u32(F.f, &d) || d == 0 || d > (1u << 20))
+        return io_fail("bad vecs record dimension", path);
+    const long long rec = 4 + (long long)d * elem;
+    if (sz % rec != 0)
+        return io_fail("vecs size is not a whole number of records", path);
+    if (num)
+        *num = (uint32_t)(sz / rec);
+    if (dim)
+        *dim = d;
+    return SS_OK;
+}
+
+template <typename Out, typename In>
+static int vecs_read(const char *path, uint32_t elem, Out *data, uint32_t num, uint32_t dim)
+{
+    uint32_t n = 0, d = 0;
+    int rc = vecs_header(path, elem, &n, &d);
+    if (rc)
+        return rc;
+    if (n != num || d != dim || (!data && num)) {
+        set_error("vecs read: size mismatch or NULL buffer");
+        return SS_EINVAL;
+    }
+    File F(path, "rb");
+    if (!F.f)
+        return io_fail("cannot open vecs file", path);
+    std::vector<In> row(d);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t di = 0;
+        if (!read_u32(F.f, &di) || di != d)
+            return io_fail(("vecs record " + std::to_string(i) + " has another dimension").c_str(), path);
+        if (std::fread(row.data(), sizeof(In), d, F.f) != d)
+            return io_fail("short vecs record", path);
+        for (uint32_t j = 0; j < d; j++)
+            data[(size_t)i * d + j] = (Out)row[j];
+    }
+    return SS_OK;
+}
+
+template <typename T>
+static int vecs_write(const char *path, const T *data, uint32_t num, uint32_t dim)
+{
+    File F(path, "wb");
+    if (!F.f)
+        return io_fail("cannot create vecs file", path);
+    for (uint32_t i = 0; i < num; i++)
+        if (std::fwrite(&dim, 4, 1, F.f) != 1 || std::fwrite(data + (size_t)i * dim, sizeof(T), dim, F.f) != dim)
+            return io_fail("write failed", path);
+    return SS_OK;
+}
+
 extern "C" {
 
 int ss_fbin_header(const char *path, uint32_t *num, uint32_t *dim)
@@ -198,6 +271,36 @@ int ss_fbin_write(const char *path, const float *data, uint32_t num, uint32_t di
         std::fwrite(data, sizeof(float), (size_t)num * dim, F.f) != (size_t)num * dim)
         return io_fail("write failed", path);
     return SS_OK;
+}
+
+int ss_vecs_header(const char *path, uint32_t elem_size, uint32_t *num, uint32_t *dim)
+{
+    return vecs_header(path, elem_size, num, dim);
+}
+
+int ss_fvecs_read(const char *path, float *data, uint32_t num, uint32_t dim)
+{
+    return vecs_read<float, float>(path, 4, data, num, dim);
+}
+
+int ss_ivecs_read(const char *path, uint32_t *data, uint32_t num, uint32_t dim)
+{
+    return vecs_read<uint32_t, uint32_t>(path, 4, data, num, dim);
+}
+
+int ss_bvecs_read(const char *path, float *data, uint32_t num, uint32_t dim)
+{
+    return vecs_read<float, uint8_t>(path, 1, data, num, dim);
+}
+
+int ss_fvecs_write(const char *path, const float *data, uint32_t num, uint32_t dim)
+{
+    return vecs_write(path, data, num, dim);
+}
+
+int ss_ivecs_write(const char *path, const uint32_t *data, uint32_t num, uint32_t dim)
+{
+    return vecs_write(path, data, num, dim);
 }
 
 int ss_gt_header(const char *path, uint32_t *nq, uint32_t *k)
