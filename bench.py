@@ -68,6 +68,8 @@ def main():
                     help="only build, then run --steps batched launches (for rocprofv3)")
     ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000",
                     help="Poisson arrival rates (QPS) of the serving sweep; empty to skip")
+    ap.add_argument("--batch-sizes", default="",
+                    help="also time hipGraph-replayed launches of these batch sizes (e.g. 1,64,1024,10000)")
     ap.add_argument("--serve-seconds", type=float, default=0.5,
                     help="length of the arrival stream at each serving rate (at least 20,000 requests)")
     args = ap.parse_args()
@@ -101,7 +103,8 @@ def main():
                  "cached": args.graph_cache}
         log(f"graph read from {args.graph_cache}, ep {ep}")
     else:
-        adj, ep, ginfo = gpu_build_graph(ix, base, w.R, w.knn_k, w.nav_L, log=log)
+        adj, ep, ginfo = gpu_build_graph(ix, base, w.R, w.knn_k, w.nav_L, log=log,
+                                         nn_descent_iters=w.extra.get("nn_descent_iters", 0))
         log(f"graph built in {time.time() - t0:.1f}s, ep {ep}")
         if args.graph_cache:
             ssio.write_nsg(args.graph_cache, adj, ep)
@@ -234,11 +237,38 @@ def main():
     plain_qps = npq / (time.perf_counter() - t)
     log(f"plain per-query launches: {plain_qps:.0f} QPS over {npq} queries")
 
+    # Launches of several batch sizes (persistent, hipGraph-replayed), at L.
+    batches = []
+    for bsz in [int(x) for x in args.batch_sizes.split(",") if x]:
+        bsz = min(bsz, nq)
+
+        def launch_b(n=bsz):
+            ix.search_device(dq.data_ptr(), n, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
+                             d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=True)
+
+        reps = max(3, min(200, 20000 // bsz))
+        launch_b()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        t_b = time.perf_counter()
+        for a_ev, b_ev in evs:
+            a_ev.record(stream)
+            launch_b()
+            b_ev.record(stream)
+        torch.cuda.synchronize()
+        wall_b = time.perf_counter() - t_b
+        kms = [a_ev.elapsed_time(b_ev) for a_ev, b_ev in evs]
+        row = dict(batch=bsz, launches=reps, qps=round(bsz * reps / wall_b, 1),
+                   launch_ms_p50=round(float(np.percentile(kms, 50)), 4),
+                   launch_ms_p99=round(float(np.percentile(kms, 99)), 4))
+        batches.append(row)
+        log(f"batch {row}")
+
     # Relaxed variants, judged by recall: wider expansion (ModifiedDeltaStepping)
     # and several wavefronts per query (ScatterSearch).
     variants = []
     if not args.no_variants:
-        for ew, wq in ((2, 1), (4, 1), (1, 2), (1, 4), (1, 8)):
+        for ew, wq in ((2, 1), (4, 1), (1, 2), (1, 4)):
             rows = []
             for Lv in sorted(set([32, 40, 48, 64, L])):
                 slots = 8192 if wq > 1 else 0
@@ -290,6 +320,7 @@ def main():
         "speedup_vs_plain": round(qps / plain_qps, 2),
         "serving": serving,
         "variants": variants,
+        "batches": batches,
         "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
                        "service_p99": round(metrics.percentile(service_ms, 0.99), 4),
                        "batch_completion_p50": round(metrics.percentile(done_ms, 0.5), 4),

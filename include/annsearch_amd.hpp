@@ -120,7 +120,7 @@ class ANNSearch {
             indices[i] = Neighbor(ids[i], d[i], false);
     }
 
-    // src/annsearch.cpp:839-1025 (ScatterSearch): num_threads wavefronts (2..8)
+    // src/annsearch.cpp:839-1025 (ScatterSearch): num_threads wavefronts (2..4)
     // search the query together, sharing the visited set; indices holds K
     // entries.  Judged by recall: the result depends on timing.
     void MultiThreadSearchArraySimulationWithET(const float *query, unsigned query_id, int K, int L,

@@ -62,7 +62,7 @@ typedef struct ss_search_params {
                                  together (ModifiedDeltaStepping,
                                  src/annsearch.cpp:1586-1731): fewer dependent
                                  steps, recall instead of exact parity */
-    uint32_t waves_per_query; /* 1 (default): one wavefront per query; 2..8:
+    uint32_t waves_per_query; /* 1 (default): one wavefront per query; 2..4:
                                  ScatterSearch (MultiThreadSearchArraySimulation-
                                  WithET, src/annsearch.cpp:839-1025): that many
                                  wavefronts per query sharing the visited set,
@@ -165,6 +165,15 @@ int ss_bruteforce_device(ss_index *index, const float *d_queries, uint32_t nq, u
 /* k-NN graph of the base itself (each point's k nearest other points),
  * written to host arrays [num][k]. */
 int ss_knn_graph(ss_index *index, uint32_t k, uint32_t *out_ids, float *out_dists);
+
+/* Approximate k-NN lists of the base by NN-descent on the GPU (Dong et al.,
+ * WWW 2011, pull form), for bases too large for ss_knn_graph: random initial
+ * lists (seeded), then `iters` rounds in which every point merges the lists
+ * of its new and changed neighbours and of up to 16 points that recently
+ * added it.  out_ids/out_dists [num][k] sorted by (distance, id); k <= 64;
+ * out_updates [iters] (may be NULL) counts list insertions per round. */
+int ss_nn_descent(ss_index *index, uint32_t k, uint32_t iters, uint32_t seed, uint32_t *out_ids,
+                  float *out_dists, uint32_t *out_updates);
 
 /* Autotuner: times launch configurations (expansion width, waves per block,
  * visited-hash size) for pool size L on a sample of queries and keeps the

@@ -118,6 +118,7 @@ _SIGS = {
     "ss_bruteforce": (cint, [vp, f32p, u32, u32, u32p, u32p, f32p]),
     "ss_bruteforce_device": (cint, [vp, vp, u32, u32, cint, u32, vp, vp, vp]),
     "ss_knn_graph": (cint, [vp, u32, u32p, f32p]),
+    "ss_nn_descent": (cint, [vp, u32, u32, u32, u32p, f32p, u32p]),
     "ss_autotune": (cint, [vp, f32p, u32, u32, u32, ctypes.c_float, ctypes.POINTER(SearchParams),
                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "ss_serve_start": (cint, [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]),

@@ -459,8 +459,10 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_EINVAL;
     }
     const uint32_t wq = p && p->waves_per_query ? p->waves_per_query : 1;
-    if (wq < 1 || wq > ss::kMaxWavesPerQueryHost || (wq > 1 && ew != 1)) {
-        set_error("waves_per_query must be 1..8, and expand_width 1 when it is above 1");
+    // 8 wavefronts per query are refused: that configuration was followed by a
+    // GPU fault in a batched run whose cause has not been found.
+    if (wq < 1 || wq > ss::kMaxWavesPerQueryEnabled || (wq > 1 && ew != 1)) {
+        set_error("waves_per_query must be 1..4, and expand_width 1 when it is above 1");
         return SS_EINVAL;
     }
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
@@ -898,6 +900,77 @@ int ss_knn_graph(ss_index *ix, uint32_t k, uint32_t *out_ids, float *out_dists)
     }
     std::lock_guard<std::mutex> lk(ix->mu);
     return bruteforce_host(ix, ix->d_base, ix->num, k, 1, out_ids, out_dists);
+}
+
+// Approximate k-NN lists by NN-descent (pull form, see ss_kernels.hip): random
+// initial lists, then `iters` iterations; out_updates[i] (optional) = list
+// insertions in iteration i.  Lists come out sorted by (distance, id).
+int ss_nn_descent(ss_index *ix, uint32_t k, uint32_t iters, uint32_t seed, uint32_t *out_ids, float *out_dists,
+                  uint32_t *out_updates)
+{
+    if (!ix || !out_ids) {
+        set_error("ss_nn_descent: NULL argument");
+        return SS_EINVAL;
+    }
+    if (k == 0 || k > ss::kNndMaxKHost || k >= ix->num) {
+        set_error("ss_nn_descent: need 1 <= k <= 64 and k < num");
+        return SS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    const size_t n = ix->num, nk = n * k;
+    const uint32_t rev_cap = 16, slots = 4096;
+    hipError_t e;
+    DevBuf<uint32_t> ids, rev, rev_cnt, ctr;
+    DevBuf<float> dists;
+    DevBuf<uint8_t> ch0, ch1;
+    if ((e = ids.reserve(nk)) || (e = dists.reserve(nk)) || (e = rev.reserve(n * rev_cap)) ||
+        (e = rev_cnt.reserve(n)) || (e = ctr.reserve(64)) || (e = ch0.reserve(n)) || (e = ch1.reserve(n)))
+        return hip_fail(e, "ss_nn_descent: allocation");
+    ss::NndArgs a{};
+    a.base = ix->d_base;
+    a.num = ix->num;
+    a.dim_pad = ix->dim_pad;
+    a.k = k;
+    a.ids = ids.ptr;
+    a.dists = dists.ptr;
+    a.rev = rev.ptr;
+    a.rev_cnt = rev_cnt.ptr;
+    a.rev_cap = rev_cap;
+    a.hash_slots = slots;
+    a.hash_shift = 32u - log2u(slots);
+    a.seed = seed;
+    a.updates = ctr.ptr;
+    a.queue = ctr.ptr + 16;
+    uint8_t *flags_prev = ch1.ptr, *flags_next = ch0.ptr;  // the initial lists mark every point changed
+    a.changed = flags_prev;
+    a.changed_next = flags_next;
+    const size_t lds_wave = ss::nnd_lds_bytes_host(ix->dim_pad, slots);
+    const uint32_t waves_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(ix->lds_per_cu / lds_wave, 16));
+    const uint32_t blocks = (uint32_t)ix->num_cus * waves_cu;
+    hipStream_t st = ix->stream;
+    if ((e = ss::launch_nnd(ix->metric, a, 0, blocks, st)) != hipSuccess)
+        return hip_fail(e, "ss_nn_descent: initial lists");
+    for (uint32_t it = 0; it < iters; it++) {
+        std::swap(flags_prev, flags_next);  // the flags written last are read now
+        a.changed = flags_prev;
+        a.changed_next = flags_next;
+        if ((e = ss::launch_clear_u32(rev_cnt.ptr, n, st)) || (e = ss::launch_nnd_reverse(a, st)) ||
+            (e = ss::launch_clear_u32(ctr.ptr, 64, st)) || (e = ss::launch_nnd(ix->metric, a, 1, blocks, st)))
+            return hip_fail(e, "ss_nn_descent: iteration");
+        uint32_t upd = 0;
+        if ((e = hipMemcpyAsync(&upd, ctr.ptr, 4, hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+            return hip_fail(e, "ss_nn_descent: progress");
+        if (out_updates)
+            out_updates[it] = upd;
+    }
+    std::vector<uint32_t> h(nk);
+    if ((e = hipMemcpyAsync(h.data(), ids.ptr, nk * 4, hipMemcpyDeviceToHost, st)) ||
+        (out_dists && (e = hipMemcpyAsync(out_dists, dists.ptr, nk * 4, hipMemcpyDeviceToHost, st))) ||
+        (e = hipStreamSynchronize(st)))
+        return hip_fail(e, "ss_nn_descent: download");
+    for (size_t i = 0; i < nk; i++)
+        out_ids[i] = h[i] == ss::kNone ? ss::kNone : (h[i] & ~ss::kExplored);
+    return SS_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -70,6 +70,24 @@ struct ServeArgs {
     uint64_t refresh_ticks; // least wall-clock ticks between reads of host memory
 };
 
+// NN-descent k-NN construction (see ss_kernels.hip).
+struct NndArgs {
+    const float *base;  // [num][dim_pad]
+    uint32_t num, dim_pad, k;
+    uint32_t *ids;       // [num][k], ascending distance, bit 31 = new
+    float *dists;        // [num][k]
+    const uint8_t *changed;  // [num] list changed in the previous iteration
+    uint8_t *changed_next;   // [num]
+    uint32_t *rev;           // [num][rev_cap]
+    uint32_t *rev_cnt;       // [num]
+    uint32_t rev_cap;
+    uint32_t hash_slots, hash_shift;
+    uint32_t seed;
+    uint32_t *updates;       // insertions in this iteration
+    uint32_t *queue;         // work counter
+};
+constexpr uint32_t kNndMaxKHost = 64;
+
 struct BruteArgs {
     const float *base;
     const float *queries;
@@ -89,7 +107,8 @@ size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash
 // LDS bytes of one block (one query) of the multi-wavefront search.
 size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
                                    uint32_t waves_per_query);
-constexpr uint32_t kMaxWavesPerQueryHost = 8;
+constexpr uint32_t kMaxWavesPerQueryHost = 8;  // LDS control arrays are sized for this
+constexpr uint32_t kMaxWavesPerQueryEnabled = 4;  // accepted by the host (see choose_shape)
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
@@ -97,6 +116,11 @@ hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
 hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t waves_per_block,
                         hipStream_t stream);
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream);
+// what: 0 = random initial lists, 1 = one pull iteration
+hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, hipStream_t stream);
+hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream);
+hipError_t launch_clear_u32(uint32_t *p, size_t n, hipStream_t stream);
+size_t nnd_lds_bytes_host(uint32_t dim_pad, uint32_t hash_slots);
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream);
 hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t dim,
                            uint32_t dim_pad, hipStream_t stream);

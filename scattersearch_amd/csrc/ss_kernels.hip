@@ -1532,6 +1532,243 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(Se
 }
 
 // ---------------------------------------------------------------------------
+// NN-descent k-NN graph (graph construction for large bases; not on the
+// query path).  Dong, Charikar, Li, "Efficient k-nearest neighbor graph
+// construction for generic similarity measures", WWW 2011, in its "pull"
+// form: in each iteration every point u merges into its own list the
+// neighbour lists of (a) its list entries that are new since u last looked,
+// (b) its list entries whose own lists changed in the previous iteration,
+// and (c) the points that recently added u (reverse neighbours, at most
+// rev_cap of them).  Each point's list is only written by its own
+// wavefront; the lists it reads may be changing concurrently, which can only
+// change which candidates it sees.  Lists hold ids with bit 31 = new.
+// ---------------------------------------------------------------------------
+
+
+__host__ __device__ inline size_t nnd_lds_bytes(uint32_t dim_pad, uint32_t hash_slots)
+{
+    return align16(dim_pad * 4u) + align16(hash_slots * 4u);
+}
+
+// Seen-set insert that never reports an id twice: an id not placed within
+// kMaxProbe probes counts as seen (the candidate is skipped).
+__device__ __forceinline__ bool nnd_first_sight(uint32_t *tab, uint32_t shift, uint32_t mask, uint32_t id)
+{
+    uint32_t s = (id * 0x9E3779B1u) >> shift;
+    for (uint32_t p = 0; p < kMaxProbe; p++) {
+        const uint32_t old = atomicCAS(&tab[s], kNone, id);
+        if (old == kNone)
+            return true;
+        if (old == id)
+            return false;
+        s = (s + 1u) & mask;
+    }
+    return false;
+}
+
+// Lane i < k holds list entry i as (distance, id, new); inserts (d, c) if it
+// beats the k-th entry, ties broken by id.  Returns whether it was inserted.
+__device__ __forceinline__ bool nnd_insert(float &ld, uint32_t &lid, bool &lnew, uint32_t k, float d, uint32_t c)
+{
+    const uint32_t lane = lane_id();
+    const bool below = lane < k && (ld < d || (ld == d && lid < c));
+    const uint32_t p = (uint32_t)__popcll(ballot(below));
+    if (p >= k)
+        return false;
+    const float pd = __shfl_up(ld, 1, 64);
+    const uint32_t pid = (uint32_t)__shfl_up((int)lid, 1, 64);
+    const bool pnew = __shfl_up((int)lnew, 1, 64) != 0;
+    if (lane < k) {
+        if (lane == p) {
+            ld = d;
+            lid = c;
+            lnew = true;
+        } else if (lane > p) {
+            ld = pd;
+            lid = pid;
+            lnew = pnew;
+        }
+    }
+    return true;
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(64) void nnd_init_kernel(NndArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *qry = reinterpret_cast<float *>(smem);
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    for (uint32_t u = blockIdx.x; u < a.num; u += gridDim.x) {
+        for (uint32_t i = lane; i < a.dim_pad; i += 64)
+            qry[i] = a.base[(size_t)u * a.dim_pad + i];
+        wave_sync();
+        // 64 pseudo-random candidates, repeats and u itself removed
+        uint32_t c = mix32(a.seed ^ mix32(u * 64u + lane)) % a.num;
+        bool ok = c != u;
+        for (uint32_t j = 0; j < 64; j++) {
+            const uint32_t cj = readlane_u(c, j);
+            ok = ok && !(j < lane && cj == c);
+        }
+        float d = __builtin_inff();
+        if (ok)
+            d = lane_dist<NBLK, TAILI, METRIC>(a.base + (size_t)c * a.dim_pad, qry, nblk_rt, tail_rt);
+        float ld = __builtin_inff();
+        uint32_t lid = kNone;
+        bool lnew = false;
+        uint64_t m = ballot(ok);
+        while (m) {
+            const uint32_t j = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            m &= m - 1ull;
+            nnd_insert(ld, lid, lnew, a.k, readlane_f(d, j), readlane_u(c, j));
+        }
+        if (lane < a.k) {
+            a.ids[(size_t)u * a.k + lane] = lid == kNone ? kNone : (lid | kExplored);
+            a.dists[(size_t)u * a.k + lane] = ld;
+        }
+        if (lane == 0)
+            a.changed_next[u] = 1;
+        wave_sync();
+    }
+}
+
+// Reverse links of the new list entries: v gets u when u's list holds v as
+// new (first come, at most rev_cap per point).
+__global__ void nnd_reverse_kernel(NndArgs a)
+{
+    const size_t total = (size_t)a.num * a.k;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t e = a.ids[i];
+        if (e == kNone || !(e & kExplored))
+            continue;
+        const uint32_t v = e & ~kExplored;
+        const uint32_t slot = atomicAdd(&a.rev_cnt[v], 1u);
+        if (slot < a.rev_cap)
+            a.rev[(size_t)v * a.rev_cap + slot] = (uint32_t)(i / a.k);
+    }
+}
+
+__global__ void nnd_clear_kernel(uint32_t *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(64) void nnd_pull_kernel(NndArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *qry = reinterpret_cast<float *>(smem);
+    uint32_t *hash = reinterpret_cast<uint32_t *>(smem + align16(a.dim_pad * 4u));
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    const uint32_t hmask = a.hash_slots - 1u;
+    while (true) {
+        uint32_t u = 0;
+        if (lane == 0)
+            u = atomicAdd(a.queue, 1u);
+        u = uniform(u);
+        if (u >= a.num)
+            break;
+        for (uint32_t i = lane; i < a.dim_pad; i += 64)
+            qry[i] = a.base[(size_t)u * a.dim_pad + i];
+        for (uint32_t i = lane; i < a.hash_slots; i += 64)
+            hash[i] = kNone;
+        wave_sync();
+        // the list, in lanes 0..k-1; it and u are seen
+        float ld = __builtin_inff();
+        uint32_t lid = kNone;
+        bool was_new = false;
+        if (lane < a.k) {
+            const uint32_t e = a.ids[(size_t)u * a.k + lane];
+            if (e != kNone) {
+                lid = e & ~kExplored;
+                was_new = (e & kExplored) != 0;
+                ld = a.dists[(size_t)u * a.k + lane];
+            }
+        }
+        bool lnew = false;  // inserted in this iteration
+        if (lane == 0)
+            nnd_first_sight(hash, a.hash_shift, hmask, u);
+        wave_sync();
+        if (lid != kNone)
+            nnd_first_sight(hash, a.hash_shift, hmask, lid);
+        // sources: new entries, entries whose lists changed, reverse links
+        const bool src_list = lid != kNone && (was_new || a.changed[lid]);
+        const uint32_t nrev = min(a.rev_cnt[u], a.rev_cap);
+        const uint32_t rsrc = lane < nrev ? a.rev[(size_t)u * a.rev_cap + lane] : kNone;
+        uint64_t ms = ballot(src_list);
+        uint64_t mr = ballot(rsrc != kNone);
+        uint32_t inserted = 0;
+        wave_sync();
+        while (ms | mr) {
+            // up to two sources per round: 2 x k <= 128 candidates, 64 per pass
+            uint32_t src[2] = {kNone, kNone};
+            for (int s = 0; s < 2; s++) {
+                if (ms) {
+                    const uint32_t j = (uint32_t)__ffsll((unsigned long long)ms) - 1u;
+                    ms &= ms - 1ull;
+                    src[s] = readlane_u(lid, j);
+                } else if (mr) {
+                    const uint32_t j = (uint32_t)__ffsll((unsigned long long)mr) - 1u;
+                    mr &= mr - 1ull;
+                    src[s] = readlane_u(rsrc, j);
+                }
+            }
+            for (int s = 0; s < 2; s++) {
+                if (src[s] == kNone)
+                    continue;
+                for (uint32_t j0 = 0; j0 < a.k; j0 += 64) {
+                    const uint32_t j = j0 + lane;
+                    uint32_t c = kNone;
+                    if (j < a.k) {
+                        const uint32_t e = a.ids[(size_t)src[s] * a.k + j];
+                        if (e != kNone)
+                            c = e & ~kExplored;
+                    }
+                    const bool fresh = c != kNone && c < a.num && nnd_first_sight(hash, a.hash_shift, hmask, c);
+                    float d = __builtin_inff();
+                    if (fresh)
+                        d = lane_dist<NBLK, TAILI, METRIC>(a.base + (size_t)c * a.dim_pad, qry, nblk_rt, tail_rt);
+                    const float kth = readlane_f(ld, a.k - 1);
+                    uint64_t m = ballot(fresh && (d < kth || (d == kth && c < readlane_u(lid, a.k - 1))));
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        inserted += nnd_insert(ld, lid, lnew, a.k, readlane_f(d, i), readlane_u(c, i)) ? 1u : 0u;
+                    }
+                }
+            }
+        }
+        if (lane < a.k) {
+            a.ids[(size_t)u * a.k + lane] = lid == kNone ? kNone : (lnew ? (lid | kExplored) : lid);
+            a.dists[(size_t)u * a.k + lane] = ld;
+        }
+        if (lane == 0) {
+            a.changed_next[u] = inserted ? 1 : 0;
+            if (inserted)
+                atomicAdd(a.updates, inserted);
+        }
+        wave_sync();
+    }
+}
+
+template <int NBLK, int TAILI, int METRIC>
+struct NndLaunch {
+    static hipError_t run(const NndArgs &a, int what, uint32_t blocks, hipStream_t st)
+    {
+        if (what == 0)
+            hipLaunchKernelGGL((nnd_init_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64), a.dim_pad * 4, st, a);
+        else
+            hipLaunchKernelGGL((nnd_pull_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64),
+                               nnd_lds_bytes(a.dim_pad, a.hash_slots), st, a);
+        return hipGetLastError();
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Brute force: a block holds kBfQ queries in LDS and streams the base in
 // tiles of 256 points, one point per lane.  Distances below a query's current
 // k-th (distance, id) go to a per-query candidate list; after each tile one
@@ -1927,6 +2164,27 @@ hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t
     hipLaunchKernelGGL(pad_copy_kernel, dim3(1024), dim3(256), 0, stream, src, dst, rows, dim, dim_pad);
     return hipGetLastError();
 }
+
+hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, hipStream_t stream)
+{
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<NndLaunch, SS_METRIC_INNER_PRODUCT>(a.dim_pad, a, what, blocks, stream);
+    return dispatch_dim<NndLaunch, SS_METRIC_L2>(a.dim_pad, a, what, blocks, stream);
+}
+
+hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream)
+{
+    hipLaunchKernelGGL(nnd_reverse_kernel, dim3(2048), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_u32(uint32_t *p, size_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(nnd_clear_kernel, dim3(1024), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+size_t nnd_lds_bytes_host(uint32_t dim_pad, uint32_t hash_slots) { return nnd_lds_bytes(dim_pad, hash_slots); }
 
 hipError_t init_kernel_attributes(size_t max_lds)
 {

@@ -46,6 +46,9 @@ WORKLOADS = {
     "c2": Workload("mixture-1.2M-d200-ip-R64", 1_200_000, 200, 10_000, 1000, 0.25,
                    METRIC_INNER_PRODUCT, 64, 64, 10, (), SEED_BASE + 2, lognormal_sigma=0.5,
                    unit_norm=True),
+    # 10M x 96, 4096 components, sigma 0.2, 100k queries, k-NN by 10 NN-descent rounds
+    "c3": Workload("mixture-10M-d96-R32", 10_000_000, 96, 100_000, 4096, 0.2, METRIC_L2, 32, 32, 10,
+                   (64, 128), SEED_BASE + 3, extra={"nn_descent_iters": 10}),
 }
 
 

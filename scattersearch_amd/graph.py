@@ -88,10 +88,17 @@ def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.0, pas
 
 
 def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.0, passes=3, chunk=131072, nthreads=0,
-                    log=None):
-    """build_graph with the GPU providers of an Index over `base`."""
+                    log=None, nn_descent_iters=0):
+    """build_graph with the GPU providers of an Index over `base`.  The k-NN
+    lists are exact (brute force) unless nn_descent_iters > 0, in which case
+    they come from that many NN-descent rounds."""
 
     def knn_fn(k):
+        if nn_descent_iters:
+            ids, _, upd = index.nn_descent(k, nn_descent_iters)
+            if log:
+                log(f"nn-descent: insertions per round {upd.tolist()}")
+            return ids
         return index.knn_graph(k)[0]
 
     def search_fn(adj, ep, L, path_cap):

@@ -207,6 +207,15 @@ class Index:
         check(lib.ss_knn_graph(self._h, k, uptr(ids), fptr(dists)))
         return ids, dists
 
+    def nn_descent(self, k, iters=10, seed=20261203):
+        """Approximate k-NN lists by NN-descent on the GPU; returns (ids,
+        dists, insertions per iteration)."""
+        ids = np.empty((self.num, k), dtype=np.uint32)
+        dists = np.empty((self.num, k), dtype=np.float32)
+        upd = np.zeros(max(iters, 1), dtype=np.uint32)
+        check(lib.ss_nn_descent(self._h, k, iters, seed, uptr(ids), fptr(dists), uptr(upd)))
+        return ids, dists, upd[:iters]
+
 
 class Metric:
     """enum Metric (include/annsearch.h:22-26)."""

@@ -56,3 +56,25 @@ def test_bruteforce_fewer_points_than_k():
     ids, d = ix.bruteforce(base[:2], 8)
     assert (ids[:, 5:] == 0xFFFFFFFF).all() and np.isinf(d[:, 5:]).all()
     assert sorted(ids[0, :5].tolist()) == list(range(5))
+
+
+def test_nn_descent_lists(small):
+    """NN-descent k-NN lists: sorted, no self, no repeats, every distance the
+    exact one, and close to the exact k-NN lists."""
+    import oracle
+    from scattersearch_amd import Index
+    base = small["base"]
+    ix = Index(base)
+    k = 16
+    ids, ds, upd = ix.nn_descent(k, iters=8, seed=5)
+    assert upd.shape == (8,) and upd[0] > 0
+    exact, _ = ix.knn_graph(k)
+    hit = 0
+    for u in range(base.shape[0]):
+        row = ids[u]
+        assert u not in row and len(set(row.tolist())) == k
+        assert (np.diff(ds[u]) >= 0).all()
+        for j in (0, k - 1):
+            assert np.float32(oracle.dist(0, base[row[j]], base[u])).view(np.uint32) == ds[u, j].view(np.uint32)
+        hit += len(set(row.tolist()) & set(exact[u].tolist()))
+    assert hit / (base.shape[0] * k) > 0.9
