@@ -482,24 +482,30 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     uint32_t t = st.t;
     uint32_t k = 0;
     bool have = false;
+    uint32_t n0 = kNone;
     for (uint32_t c0 = 0; c0 < t; c0 += 64) {  // first unexplored entry
+        // The id is taken from the value the ballot tested, never read again:
+        // with several wavefronts per query another wavefront may set the
+        // explored bit of this entry at any time (steal_entries), and a second
+        // read would return the id with bit 31 set.
         const uint32_t i = c0 + lane;
-        const uint64_t m = ballot(i < t && !(w.pool_id[i] & kExplored));
+        const uint32_t v = i < t ? w.pool_id[i] : kNone;
+        const uint64_t m = ballot(!(v & kExplored));
         if (m) {
-            k = c0 + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            k = c0 + l;
+            n0 = readlane_u(v, l) & ~kExplored;
             have = true;
             break;
         }
     }
     uint32_t nb = kNone;
-    if (have) {
-        const uint32_t n0 = uniform(w.pool_id[k]);
+    if (have)
         nb = lane < a.r_pad ? a.adj[(size_t)n0 * a.r_pad + lane] : kNone;
-    }
     while (have) {
         if (!hook(st, t))
             break;
-        const uint32_t n = uniform(w.pool_id[k]);
+        const uint32_t n = uniform(w.pool_id[k]) & ~kExplored;  // a thief may have marked it
         const float tail = readlane_f(w.pool_d[t - 1], 0);
         wave_sync();
         if (lane == 0) {

@@ -73,6 +73,29 @@ def test_scatter_repeated_launches_stay_valid(small):
         _check_entries(small["base"], q, ids, ds, 10)
 
 
+@pytest.mark.parametrize("W", [2, 4])
+def test_scatter_long_batches_stay_in_bounds(c0, W):
+    """Regression: a wavefront whose entries were being stolen used to read a
+    pool id a second time after the ballot that found it unexplored and could
+    get it back with the explored bit set (an adjacency load far out of
+    bounds).  Long replayed batches make thefts frequent; every entry must
+    stay a real point at its distance."""
+    base = c0["base"]
+    rng = np.random.default_rng(11)
+    q = base[rng.choice(base.shape[0], 10000, replace=False)] + np.float32(0.05) * rng.standard_normal(
+        (10000, base.shape[1]), dtype=np.float32)
+    ix = _index(c0)
+    for rep in range(4):
+        L = (32, 48, 64, 48)[rep]
+        ids, ds, st = ix.search(q, 10, L, waves_per_query=W, visited_slots=8192, use_graph=rep > 0)
+        assert (ids < base.shape[0]).all(), rep
+        assert all(len(set(r.tolist())) == 10 for r in ids), rep
+        assert (np.diff(ds, axis=1) >= 0).all(), rep
+        want = ((base[ids].astype(np.float64) - q[:, None, :].astype(np.float64)) ** 2).sum(-1)
+        np.testing.assert_allclose(ds, want, rtol=1e-5)
+    _check_entries(base, q[:50], ids[:50], ds[:50], 10)
+
+
 def test_scatter_per_query_launches(small):
     from scattersearch_amd import SS_MODE_PER_QUERY
     ix = _index(small)
