@@ -14,10 +14,8 @@
 // and the pairwise sum ((s0+s1)+(s2+s3))+((s4+s5)+(s6+s7)).  The file is
 // compiled with -ffp-contract=off so nothing else gets fused.
 //
-// In the search a team of two lanes computes one distance: lane h of the team
-// owns partials 4h..4h+3 and reads them with 16-byte loads (floats
-// 16b+4h..16b+4h+3 and 16b+8+4h..16b+8+4h+3 of every block b), so one
-// wavefront gathers 32 neighbour vectors at once with coalesced dwordx4 loads.
+// In the search a team of four lanes computes a distance (team_dists), so one
+// wavefront gathers 16 or 32 neighbour vectors per pass.
 
 #include "ss_internal.h"
 
@@ -61,107 +59,177 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
     return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
 }
 
+// v of another lane of the same row of 16, chosen by a DPP control (all lanes
+// involved active).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // ---------------------------------------------------------------------------
-// Team distance (two lanes per vector).  NBLK < 0 selects the runtime-size
+// Team distances: four lanes per vector, P vectors per lane.  Lane h of a
+// team owns partials 2h, 2h+1 and reads them with 8-byte loads (floats
+// 16b+2h, 16b+2h+1 and 16b+8+2h, 16b+8+2h+1 of every block b), so one
+// wavefront gathers 16 vectors per pass with every lane busy; with P = 2 a
+// lane computes the same partials of two vectors and reads its share of the
+// query once for both.  The pairwise sum is rebuilt across the team:
+// (s0+s1)+(s2+s3) from lanes 0 and 1, then + ((s4+s5)+(s6+s7)) from lanes 2
+// and 3; IEEE addition commutes exactly, so every lane of the team ends with
+// the reference's bits.  All loads of a group of blocks are issued before
+// their first use so the rows' cache lines are fetched in parallel (at most
+// 32 loads, 64 registers, in flight).  NBLK < 0 selects the runtime-size
 // form; otherwise NBLK full 16-float blocks and TAILI (0/1) leading tail
 // block are compile-time constants.
 // ---------------------------------------------------------------------------
-template <int NBLK, int TAILI, int METRIC>
-__device__ __forceinline__ float team_dist(const float *__restrict__ row, const float *q, int h,
-                                           int nblk_rt, int tail_rt)
+template <int NBLK, int TAILI, int METRIC, int P>
+__device__ __forceinline__ void team_dists(const float *const (&rows)[P], const float *q, int h, int nblk_rt,
+                                           int tail_rt, float (&out)[P])
 {
     const int nblk = NBLK >= 0 ? NBLK : nblk_rt;
     const bool tail = NBLK >= 0 ? (TAILI != 0) : (tail_rt != 0);
-    const float4 *r4 = reinterpret_cast<const float4 *>(row);
-    const float4 *q4 = reinterpret_cast<const float4 *>(q);
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    auto tail_block = [&](const float4 x) {
-        const float4 y = q4[4 * nblk + h];
+    const float2 *q2 = reinterpret_cast<const float2 *>(q);
+    const float2 *r2[P];
+    float s0[P], s1[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        r2[p] = reinterpret_cast<const float2 *>(rows[p]);
+        s0[p] = 0.f;
+        s1[p] = 0.f;
+    }
+    auto tail_block = [&](int p, const float2 x, const float2 y) {
         if (METRIC == SS_METRIC_L2) {
-            const float d0 = y.x - x.x, d1 = y.y - x.y, d2 = y.z - x.z, d3 = y.w - x.w;
-            s0 = d0 * d0;
-            s1 = d1 * d1;
-            s2 = d2 * d2;
-            s3 = d3 * d3;
+            const float d0 = y.x - x.x, d1 = y.y - x.y;
+            s0[p] = d0 * d0;
+            s1[p] = d1 * d1;
         } else {
-            s0 = y.x * x.x;
-            s1 = y.y * x.y;
-            s2 = y.z * x.z;
-            s3 = y.w * x.w;
+            s0[p] = y.x * x.x;
+            s1[p] = y.y * x.y;
         }
     };
-    auto block = [&](int b, const float4 xa, const float4 xb) {
-        const float4 qa = q4[4 * b + h];
-        const float4 qb = q4[4 * b + 2 + h];
+    auto block = [&](int p, const float2 xa, const float2 xb, const float2 qa, const float2 qb) {
         if (METRIC == SS_METRIC_L2) {
-            const float a0 = qa.x - xa.x, a1 = qa.y - xa.y, a2 = qa.z - xa.z, a3 = qa.w - xa.w;
-            const float b0 = qb.x - xb.x, b1 = qb.y - xb.y, b2 = qb.z - xb.z, b3 = qb.w - xb.w;
-            s0 = s0 + __builtin_fmaf(a0, a0, b0 * b0);
-            s1 = s1 + __builtin_fmaf(a1, a1, b1 * b1);
-            s2 = s2 + __builtin_fmaf(a2, a2, b2 * b2);
-            s3 = s3 + __builtin_fmaf(a3, a3, b3 * b3);
+            const float a0 = qa.x - xa.x, a1 = qa.y - xa.y;
+            const float b0 = qb.x - xb.x, b1 = qb.y - xb.y;
+            s0[p] = s0[p] + __builtin_fmaf(a0, a0, b0 * b0);
+            s1[p] = s1[p] + __builtin_fmaf(a1, a1, b1 * b1);
         } else {
-            s0 = s0 + __builtin_fmaf(qa.x, xa.x, qb.x * xb.x);
-            s1 = s1 + __builtin_fmaf(qa.y, xa.y, qb.y * xb.y);
-            s2 = s2 + __builtin_fmaf(qa.z, xa.z, qb.z * xb.z);
-            s3 = s3 + __builtin_fmaf(qa.w, xa.w, qb.w * xb.w);
+            s0[p] = s0[p] + __builtin_fmaf(qa.x, xa.x, qb.x * xb.x);
+            s1[p] = s1[p] + __builtin_fmaf(qa.y, xa.y, qb.y * xb.y);
         }
     };
     if constexpr (NBLK >= 0) {
-        // Issue the 16-byte loads of the row (in one group up to 8 blocks, in
-        // two halves beyond, to bound registers) before the first use, so the
-        // row's cache lines are fetched in parallel rather than one after
-        // another.
-        constexpr int G = NBLK <= 8 ? NBLK : (NBLK + 1) / 2;
-        float4 xt;
-        if (TAILI)
-            xt = r4[4 * NBLK + h];
+        constexpr int G = NBLK * P <= 16 ? NBLK : (NBLK * P <= 32 ? (NBLK + 1) / 2 : (NBLK + 3) / 4);
+        float2 xt[P];
+        if (TAILI) {
+#pragma unroll
+            for (int p = 0; p < P; p++)
+                xt[p] = r2[p][8 * NBLK + h];
+        }
 #pragma unroll
         for (int g0 = 0; g0 < NBLK; g0 += G) {
-            float4 xa[G > 0 ? G : 1], xb[G > 0 ? G : 1];
+            // The group's row loads are all issued before the first use; the
+            // query share (LDS) follows in sub-groups of QG blocks (all blocks
+            // for one vector per lane; one block at a time for two, which
+            // bounds registers), each sub-group's reads issued together.
+            constexpr int QG = P == 1 ? G : 1;
+            float2 xa[P][G > 0 ? G : 1], xb[P][G > 0 ? G : 1];
 #pragma unroll
             for (int j = 0; j < G; j++) {
-                if (g0 + j < NBLK) {
-                    xa[j] = r4[4 * (g0 + j) + h];
-                    xb[j] = r4[4 * (g0 + j) + 2 + h];
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    if (g0 + j < NBLK) {
+                        xa[p][j] = r2[p][8 * (g0 + j) + h];
+                        xb[p][j] = r2[p][8 * (g0 + j) + 4 + h];
+                    }
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
-            if (TAILI && g0 == 0)
-                tail_block(xt);
 #pragma unroll
-            for (int j = 0; j < G; j++)
-                if (g0 + j < NBLK)
-                    block(g0 + j, xa[j], xb[j]);
+            for (int s0q = 0; s0q < G; s0q += QG) {
+                float2 qa[QG > 0 ? QG : 1], qb[QG > 0 ? QG : 1], yt;
+#pragma unroll
+                for (int j = 0; j < QG; j++) {
+                    if (s0q + j < G && g0 + s0q + j < NBLK) {
+                        qa[j] = q2[8 * (g0 + s0q + j) + h];
+                        qb[j] = q2[8 * (g0 + s0q + j) + 4 + h];
+                    }
+                }
+                if (TAILI && g0 == 0 && s0q == 0)
+                    yt = q2[8 * NBLK + h];
+                __builtin_amdgcn_sched_barrier(0);
+                if (TAILI && g0 == 0 && s0q == 0) {
+#pragma unroll
+                    for (int p = 0; p < P; p++)
+                        tail_block(p, xt[p], yt);
+                }
+#pragma unroll
+                for (int j = 0; j < QG; j++) {
+                    if (s0q + j < G && g0 + s0q + j < NBLK) {
+#pragma unroll
+                        for (int p = 0; p < P; p++)
+                            block(p, xa[p][s0q + j], xb[p][s0q + j], qa[j], qb[j]);
+                    }
+                }
+            }
         }
     } else {
-        // Runtime size: four blocks (eight loads) in flight at a time.
-        if (tail)
-            tail_block(r4[4 * nblk + h]);
-        for (int b0 = 0; b0 < nblk; b0 += 4) {
-            float4 xa[4], xb[4];
+        // Runtime size: 8 / P blocks of every vector in flight at a time.
+        constexpr int G = 8 / P;
+        if (tail) {
+            const float2 y = q2[8 * nblk + h];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (b0 + j < nblk) {
-                    xa[j] = r4[4 * (b0 + j) + h];
-                    xb[j] = r4[4 * (b0 + j) + 2 + h];
+            for (int p = 0; p < P; p++)
+                tail_block(p, r2[p][8 * nblk + h], y);
+        }
+        for (int b0 = 0; b0 < nblk; b0 += G) {
+            float2 xa[P][G], xb[P][G];
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    if (b0 + j < nblk) {
+                        xa[p][j] = r2[p][8 * (b0 + j) + h];
+                        xb[p][j] = r2[p][8 * (b0 + j) + 4 + h];
+                    }
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (b0 + j < nblk)
-                    block(b0 + j, xa[j], xb[j]);
+            for (int j = 0; j < G; j++) {
+                if (b0 + j < nblk) {
+                    const float2 qa = q2[8 * (b0 + j) + h];
+                    const float2 qb = q2[8 * (b0 + j) + 4 + h];
+#pragma unroll
+                    for (int p = 0; p < P; p++)
+                        block(p, xa[p][j], xb[p][j], qa, qb);
+                }
+            }
         }
     }
-    const float part = (s0 + s1) + (s2 + s3);
-    const float other = __shfl_xor(part, 1, 64);
-    const float r = part + other;  // lane 0 of the team: low + high; lane 1: high + low (same bits)
-    return METRIC == SS_METRIC_INNER_PRODUCT ? -r : r;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        // within the quad of the team: lane ^ 1, then lane ^ 2 (DPP quad_perm)
+        const float part = s0[p] + s1[p];
+        const float pair = part + dpp_f<0xB1>(part);
+        const float r = pair + dpp_f<0x4E>(pair);
+        out[p] = METRIC == SS_METRIC_INNER_PRODUCT ? -r : r;
+    }
+}
+
+// Minimum of v over the wavefront (every lane active): DPP butterflies inside
+// each row of 16 lanes, then the four row minima.
+
+__device__ __forceinline__ float wave_min_f(float v)
+{
+    v = fminf(v, dpp_f<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fminf(v, dpp_f<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fminf(v, dpp_f<0x124>(v));  // row_ror:4
+    v = fminf(v, dpp_f<0x128>(v));  // row_ror:8
+    return fminf(fminf(readlane_f(v, 0), readlane_f(v, 16)), fminf(readlane_f(v, 32), readlane_f(v, 48)));
 }
 
 // Whole-vector distance in one lane (brute force).  q is the query, row the
-// base vector; same arithmetic as team_dist.
+// base vector; same arithmetic as team_dists.
 template <int NBLK, int TAILI, int METRIC>
 __device__ __forceinline__ float lane_dist(const float *__restrict__ row, const float *q, int nblk_rt,
                                            int tail_rt)
@@ -213,29 +281,74 @@ __device__ __forceinline__ float lane_dist(const float *__restrict__ row, const 
 }
 
 // ---------------------------------------------------------------------------
-// Visited set: open-addressed hash of node ids in LDS, linear probing.  A
-// lane that finds no free slot within kMaxProbe probes treats its id as
-// unvisited (and counts a lossy insert).  Forgetting a visited node never
-// changes the search result: its recomputed distance is either not below the
-// pool tail (the tail only decreases) or it is still in the pool, where the
-// merge drops the repeat; only the distance count grows.
+// Visited set: a hash of node ids in LDS made of 32-byte buckets of eight
+// slots.  A lookup is one read of the id's bucket (two 16-byte LDS reads);
+// an absent id claims a free slot of the bucket with a compare-and-swap,
+// starting at slot (id mod 8) so that two ids of one bucket rarely race for
+// the same slot.  The claim's outcome is checked later (visit_settle), off
+// the step's critical path: a claim that lost its slot tries the bucket's
+// other free slots.  A lane that finds its bucket full treats its id as
+// unvisited without recording it (a lossy insert, counted).  Forgetting a
+// visited node never changes the search result: its recomputed distance is
+// either not below the pool tail (the tail only decreases) or it is still in
+// the pool, where the merge drops the repeat; only the distance count grows.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool visit(uint32_t *tab, uint32_t shift, uint32_t mask, uint32_t id,
-                                      uint32_t &lossy)
+constexpr uint32_t kBucket = 8;
+
+struct VisitClaim {
+    uint32_t slot;  // claimed slot, kNone when no claim was made
+    uint32_t old;   // what the compare-and-swap found there
+};
+
+// bshift = 32 - log2(number of buckets).  Returns whether id was absent.
+__device__ __forceinline__ bool visit_begin(uint32_t *tab, uint32_t bshift, uint32_t id, VisitClaim &c,
+                                            uint32_t &lossy)
 {
-    uint32_t s = (id * 0x9E3779B1u) >> shift;
-    for (uint32_t p = 0; p < kMaxProbe; p++) {
-        // One LDS round trip per probe: claim the slot if it is empty, and
-        // learn what it holds otherwise.
-        const uint32_t old = atomicCAS(&tab[s], kNone, id);
-        if (old == kNone)
-            return true;
-        if (old == id)
-            return false;
-        s = (s + 1u) & mask;
+    c.slot = kNone;
+    c.old = kNone;
+    const uint32_t b = ((id * 0x9E3779B1u) >> bshift) * kBucket;
+    const uint4 *t4 = reinterpret_cast<const uint4 *>(tab + b);
+    const uint4 x = t4[0], y = t4[1];
+    if (x.x == id || x.y == id || x.z == id || x.w == id || y.x == id || y.y == id || y.z == id || y.w == id)
+        return false;
+    const uint32_t fm = (x.x == kNone ? 1u : 0u) | (x.y == kNone ? 2u : 0u) | (x.z == kNone ? 4u : 0u) |
+                        (x.w == kNone ? 8u : 0u) | (y.x == kNone ? 16u : 0u) | (y.y == kNone ? 32u : 0u) |
+                        (y.z == kNone ? 64u : 0u) | (y.w == kNone ? 128u : 0u);
+    if (fm == 0) {
+        lossy++;
+        return true;
+    }
+    const uint32_t s0 = id & (kBucket - 1);
+    const uint32_t rot = ((fm >> s0) | (fm << (kBucket - s0))) & 0xFFu;
+    c.slot = b + ((s0 + (uint32_t)__builtin_ctz(rot)) & (kBucket - 1));
+    c.old = atomicCAS(&tab[c.slot], kNone, id);
+    return true;
+}
+
+// Completes a claim of visit_begin: a claim that found its slot taken (by
+// another lane of this step, or another wavefront of the query) tries the
+// bucket's remaining slots.
+__device__ __forceinline__ void visit_settle(uint32_t *tab, uint32_t id, const VisitClaim &c, uint32_t &lossy)
+{
+    if (c.slot == kNone || c.old == kNone || c.old == id)
+        return;
+    const uint32_t b = c.slot & ~(kBucket - 1);
+    for (uint32_t i = 1; i < kBucket; i++) {
+        const uint32_t s = b + ((c.slot + i) & (kBucket - 1));
+        const uint32_t o = atomicCAS(&tab[s], kNone, id);
+        if (o == kNone || o == id)
+            return;
     }
     lossy++;
-    return true;
+}
+
+// Lookup and insert in one go.
+__device__ __forceinline__ bool visit(uint32_t *tab, uint32_t bshift, uint32_t id, uint32_t &lossy)
+{
+    VisitClaim c;
+    const bool fresh = visit_begin(tab, bshift, id, c, lossy);
+    visit_settle(tab, id, c, lossy);
+    return fresh;
 }
 
 struct WaveLds {
@@ -272,21 +385,28 @@ __device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_m
     return w;
 }
 
-// Distances of candidates cand_id[0..nc) into cand_d[0..nc), 32 per pass.
+// Distances of candidates cand_id[0..nc) into cand_d[0..nc), 32 per pass:
+// lane 4c+h computes candidates c and c + 16 of the pass (a missing second
+// candidate repeats the first, whose lines the same loads already fetch).
 template <int NBLK, int TAILI, int METRIC>
 __device__ __forceinline__ void candidate_distances(const SearchArgs &a, const WaveLds &w, uint32_t nc,
                                                     int nblk_rt, int tail_rt)
 {
     const uint32_t lane = lane_id();
-    const int h = lane & 1;
+    const int h = lane & 3;
     for (uint32_t c0 = 0; c0 < nc; c0 += 32) {
-        const uint32_t c = c0 + (lane >> 1);
+        const uint32_t c = c0 + (lane >> 2);
         if (c < nc) {
-            const uint32_t id = w.cand_id[c];
-            const float d = team_dist<NBLK, TAILI, METRIC>(a.base + (size_t)id * a.dim_pad, w.qry, h,
-                                                           nblk_rt, tail_rt);
-            if (h == 0)
-                w.cand_d[c] = d;
+            const uint32_t id0 = w.cand_id[c];
+            const uint32_t id1 = c + 16 < nc ? w.cand_id[c + 16] : id0;
+            const float *rows[2] = {a.base + (size_t)id0 * a.dim_pad, a.base + (size_t)id1 * a.dim_pad};
+            float d[2];
+            team_dists<NBLK, TAILI, METRIC, 2>(rows, w.qry, h, nblk_rt, tail_rt, d);
+            if (h == 0) {
+                w.cand_d[c] = d[0];
+                if (c + 16 < nc)
+                    w.cand_d[c + 16] = d[1];
+            }
         }
     }
 }
@@ -398,7 +518,7 @@ __device__ __forceinline__ void seed_pool(const SearchArgs &a, const WaveLds &w,
     const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
     const uint32_t t = deg < st.L ? deg : st.L;
     if (lane < t) {
-        visit(w.hash, a.hash_shift, a.hash_slots - 1u, nb, st.lossy);
+        visit(w.hash, a.hash_shift + 3u, nb, st.lossy);
         w.cand_id[lane] = nb;
     }
     wave_sync();
@@ -464,6 +584,14 @@ struct NoHook {
 // The expansion loop of one wavefront over its pool (see search_one): runs
 // until no unexplored entry is left or hook(st, t) returns false at the
 // start of a step.
+//
+// Latency.  A step has two dependent memory round trips: the adjacency row
+// of the node, then the vectors of its fresh neighbours.  The next node is
+// usually the first unexplored pool entry after the current one (it stays
+// first unless an admitted neighbour is closer), so that entry's row is
+// loaded at the start of the step, while the vectors are in flight; when an
+// admitted neighbour wins instead, its row is loaded as soon as the
+// smallest admitted distance is known, before the merge ranks are counted.
 template <int NBLK, int TAILI, int METRIC, int MC, typename Hook>
 __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &w, uint32_t q, QueryState &st,
                                             Hook &&hook)
@@ -472,13 +600,11 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
-    const uint32_t hmask = a.hash_slots - 1u;
+    const uint32_t bshift = a.hash_shift + 3u;  // buckets of eight slots
     const uint32_t L = st.L;
 
-    // Expansion loop.  k is the pool position of the node to expand and nb
-    // already holds its adjacency row: the next node is decided from
-    // registers before the merge is written, and its row load is issued
-    // then, so the load overlaps the merge.
+    // k is the pool position of the node to expand and nb already holds its
+    // adjacency row.
     uint32_t t = st.t;
     uint32_t k = 0;
     bool have = false;
@@ -505,8 +631,8 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     while (have) {
         if (!hook(st, t))
             break;
-        const uint32_t n = uniform(w.pool_id[k]) & ~kExplored;  // a thief may have marked it
         const float tail = readlane_f(w.pool_d[t - 1], 0);
+        const uint32_t n = uniform(w.pool_id[k]) & ~kExplored;  // a thief may have marked it
         wave_sync();
         if (lane == 0) {
             w.pool_id[k] = n | kExplored;
@@ -515,8 +641,38 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
         st.hops++;
 
+        // First unexplored entry after k (every entry up to k is explored);
+        // its row is the likely next one.  Id and distance come from the
+        // values the ballot tested.
+        int onext = -1;
+        uint32_t oid = kNone;
+        float od = __builtin_inff();
+        for (uint32_t c0 = (k + 1u) & ~63u; c0 < t; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            const bool in = i > k && i < t;
+            const uint32_t v = in ? w.pool_id[i] : kNone;
+            const float dv = in ? w.pool_d[i] : __builtin_inff();
+            const uint64_t m = ballot(!(v & kExplored));
+            if (m) {
+                const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                onext = (int)(c0 + l);
+                oid = readlane_u(v, l);
+                od = readlane_f(dv, l);
+                break;
+            }
+        }
+
+        // The row of this node is needed now; the likely next row is issued
+        // only after it has arrived (vmcnt(0): this row is the only vector
+        // memory access in flight), so that no later wait for this row also
+        // waits for that one: the counter of loads in flight is in order.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t nbs = onext >= 0 && lane < a.r_pad ? a.adj[(size_t)oid * a.r_pad + lane] : kNone;
+
         // Visited test of the whole row, compaction of the fresh ids.
-        const bool fresh = nb != kNone && visit(w.hash, a.hash_shift, hmask, nb, st.lossy);
+        VisitClaim claim{kNone, kNone};
+        const bool fresh = nb != kNone && visit_begin(w.hash, bshift, nb, claim, st.lossy);
         const uint64_t fm = ballot(fresh);
         const uint32_t nc = (uint32_t)__popcll(fm);
         if (fresh)
@@ -524,22 +680,39 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         wave_sync();
         float cd = __builtin_inff();
         uint32_t cid = kNone;
-        if (nc <= 32) {
-            // One pass: the team of lanes 2c, 2c+1 computes candidate c, and
-            // lane c collects its id and distance from lane 2c.
-            const uint32_t c = lane >> 1;
+        if (nc <= 16) {
+            // Lanes 4c..4c+3 compute candidate c; lane c collects its id and
+            // distance from lane 4c.
+            const uint32_t c = lane >> 2;
             uint32_t tid = kNone;
-            float td = __builtin_inff();
+            float td[1] = {__builtin_inff()};
             if (c < nc) {
                 tid = w.cand_id[c];
-                td = team_dist<NBLK, TAILI, METRIC>(a.base + (size_t)tid * a.dim_pad, w.qry, (int)(lane & 1),
-                                                    nblk_rt, tail_rt);
+                const float *rows[1] = {a.base + (size_t)tid * a.dim_pad};
+                team_dists<NBLK, TAILI, METRIC, 1>(rows, w.qry, (int)(lane & 3), nblk_rt, tail_rt, td);
             }
-            const float gd = __shfl(td, (int)(2 * (lane & 31)), 64);
-            const uint32_t gid = (uint32_t)__shfl((int)tid, (int)(2 * (lane & 31)), 64);
+            const float gd = __shfl(td[0], (int)(4 * (lane & 15)), 64);
+            const uint32_t gid = (uint32_t)__shfl((int)tid, (int)(4 * (lane & 15)), 64);
             if (lane < nc) {
                 cd = gd;
                 cid = gid;
+            }
+        } else if (nc <= 32) {
+            // Lanes 4c..4c+3 compute candidates c and c + 16 (a missing
+            // second candidate repeats the first); lane c collects from lane
+            // 4(c mod 16).
+            const uint32_t c = lane >> 2;
+            const uint32_t id0 = w.cand_id[c];
+            const uint32_t id1 = c + 16 < nc ? w.cand_id[c + 16] : id0;
+            const float *rows[2] = {a.base + (size_t)id0 * a.dim_pad, a.base + (size_t)id1 * a.dim_pad};
+            float td[2];
+            team_dists<NBLK, TAILI, METRIC, 2>(rows, w.qry, (int)(lane & 3), nblk_rt, tail_rt, td);
+            const int src = (int)(4 * (lane & 15));
+            const float gd0 = __shfl(td[0], src, 64), gd1 = __shfl(td[1], src, 64);
+            const uint32_t gi0 = (uint32_t)__shfl((int)id0, src, 64), gi1 = (uint32_t)__shfl((int)id1, src, 64);
+            if (lane < nc) {
+                cd = lane < 16 ? gd0 : gd1;
+                cid = lane < 16 ? gi0 : gi1;
             }
         } else {
             candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
@@ -550,6 +723,8 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             }
         }
         st.ndc += nc;
+        if (fresh)
+            visit_settle(w.hash, nb, claim, st.lossy);
         wave_sync();
         bool inA = lane < nc && cd < tail;  // strict tail filter, :238
 
@@ -566,25 +741,6 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             if ((uint32_t)c < nch && i < t) {
                 pid[c] = w.pool_id[i];
                 pdv[c] = w.pool_d[i];
-            }
-        }
-
-        // First unexplored entry after k in the pool as it is now (every entry
-        // up to k is explored).
-        int onext = -1;
-        uint32_t oid = kNone;
-        float od = __builtin_inff();
-#pragma unroll
-        for (int c = 0; c < kMaxChunks; c++) {
-            if ((uint32_t)c < nch && onext < 0) {
-                const uint32_t i = (uint32_t)c * 64u + lane;
-                const uint64_t m = ballot(i > k && i < t && !(pid[c] & kExplored));
-                if (m) {
-                    const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                    onext = c * 64 + (int)l;
-                    oid = readlane_u(pid[c], l);
-                    od = readlane_f(pdv[c], l);
-                }
             }
         }
 
@@ -608,54 +764,51 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         const uint64_t am = ballot(inA);
         if (am == 0) {  // nothing admitted: nk = L, the cursor moves on (:249-252)
             have = onext >= 0;
-            if (have) {
+            if (have)
                 k = (uint32_t)onext;
-                nb = lane < a.r_pad ? a.adj[(size_t)oid * a.r_pad + lane] : kNone;
-            }
+            nb = nbs;
             continue;
         }
         const uint32_t na = (uint32_t)__popcll(am);
 
+        // Next node: the first unexplored entry of the merged pool is either
+        // the old one found above (it keeps position onext when no admitted
+        // candidate is smaller, P first on ties) or the first smallest
+        // admitted candidate; both lie inside the first L.
+        const float dmin = wave_min_f(inA ? cd : __builtin_inff());
+        const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)ballot(inA && cd == dmin)) - 1u;
+        const bool keep_old = onext >= 0 && od <= dmin;
+        if (keep_old) {
+            nb = nbs;
+        } else {
+            const uint32_t next_id = readlane_u(cid, jmin);
+            nb = lane < a.r_pad ? a.adj[(size_t)next_id * a.r_pad + lane] : kNone;
+        }
+
         // Merge ranks.  Pool entry i moves to i + #{a in A: a < p};
-        // candidate a goes to rankA(a) + #{p in P: p <= a}.
+        // candidate a goes to rankA(a) + #{p in P: p <= a}.  Branch-free
+        // body: one pass per admitted candidate.
         uint32_t rankA = 0, posA = 0;
+        const uint32_t inA_u = inA ? 1u : 0u;
         uint64_t it = am;
         while (it) {
             const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
             it &= it - 1ull;
             const float dj = readlane_f(cd, j);
-            rankA += (inA && (dj < cd || (dj == cd && j < lane))) ? 1u : 0u;
+            rankA += inA_u & ((dj < cd ? 1u : 0u) | ((dj == cd ? 1u : 0u) & (j < lane ? 1u : 0u)));
             uint32_t le = 0;
 #pragma unroll
             for (int c = 0; c < kMaxChunks; c++) {
                 if ((uint32_t)c < nch) {
                     cnt[c] += dj < pdv[c] ? 1u : 0u;
-                    const uint32_t i = (uint32_t)c * 64u + lane;
-                    le += (uint32_t)__popcll(ballot(i < t && pdv[c] <= dj));
+                    // entries past t hold +inf and never count
+                    le += (uint32_t)__popcll(ballot(pdv[c] <= dj));
                 }
             }
-            if (lane == j)
-                posA = le;
+            posA = lane == j ? le : posA;
         }
         posA += rankA;
-        const uint64_t mm = ballot(inA && rankA == 0);
-        const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)mm) - 1u;
-        const uint32_t nk = readlane_u(posA, jmin);  // :244-245
-        const float dmin = readlane_f(cd, jmin);
-
-        // Next node: the first unexplored entry of the merged pool is either
-        // the old one found above (it keeps position onext when no admitted
-        // candidate is smaller, P first on ties) or the smallest admitted
-        // candidate at position nk; both lie inside the first L.
-        uint32_t next_id;
-        if (onext >= 0 && od <= dmin) {
-            k = (uint32_t)onext;
-            next_id = oid;
-        } else {
-            k = nk;
-            next_id = readlane_u(cid, jmin);
-        }
-        nb = lane < a.r_pad ? a.adj[(size_t)next_id * a.r_pad + lane] : kNone;
+        k = keep_old ? (uint32_t)onext : readlane_u(posA, jmin);  // :244-245
 
         wave_sync();
 #pragma unroll
@@ -714,7 +867,6 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
-    const uint32_t hmask = a.hash_slots - 1u;
     const uint32_t L = st.L;
     uint32_t t = st.t;
     while (true) {
@@ -761,7 +913,7 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
         uint32_t nc = 0;
 #pragma unroll
         for (int r = 0; r < EW; r++) {
-            const bool fresh = nb[r] != kNone && visit(w.hash, a.hash_shift, hmask, nb[r], st.lossy);
+            const bool fresh = nb[r] != kNone && visit(w.hash, a.hash_shift + 3u, nb[r], st.lossy);
             const uint64_t fm = ballot(fresh);
             if (fresh)
                 w.cand_id[nc + lanes_below(fm)] = nb[r];
