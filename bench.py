@@ -126,7 +126,10 @@ def main():
                          visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq)
 
     def stats_host():
-        return d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
+        st = d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
+        if (st["status"] == ss._lib.SS_QUERY_FAULT).any():
+            raise RuntimeError("search reported an internal error (node id outside the index)")
+        return st
 
     if args.profile:
         L = args.L or 64

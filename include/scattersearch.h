@@ -30,6 +30,7 @@ extern "C" {
 #define SS_EDEVICE (-19) /* no usable GPU or a HIP call failed */
 #define SS_ESTATE (-71)  /* call not valid in the index's current state */
 #define SS_EBUSY (-16)   /* serving queue full */
+#define SS_EINTERNAL (-131) /* internal error of the device search (see ss_query_stats.status) */
 
 #define SS_NONE 0xFFFFFFFFu
 
@@ -71,12 +72,19 @@ typedef struct ss_search_params {
     uint32_t reserved;
 } ss_search_params;
 
+#define SS_QUERY_OK 0u
+#define SS_QUERY_REJECTED 0xFFFFFFFFu
+#define SS_QUERY_FAULT 0x0BAD0001u
+
 /* Per-query counters written by every search. */
 typedef struct ss_query_stats {
     uint32_t ndc;     /* distance computations, including the seed ones */
     uint32_t hops;    /* expanded nodes */
     uint32_t lossy;   /* visited-hash inserts that found no free slot */
-    uint32_t reserved;
+    uint32_t status;  /* SS_QUERY_OK; SS_QUERY_REJECTED: K or L outside
+                         1 <= K <= L <= the launch's maximum (empty result);
+                         SS_QUERY_FAULT: the device met a node id outside the
+                         index (an internal error: the result is not valid) */
     uint64_t t_start; /* device wall clock when a wavefront took the query */
     uint64_t t_end;   /* device wall clock when its results were written */
 } ss_query_stats;

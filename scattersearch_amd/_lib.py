@@ -17,6 +17,10 @@ SS_EIO = -5
 SS_EDEVICE = -19
 SS_ESTATE = -71
 SS_EBUSY = -16
+SS_EINTERNAL = -131
+SS_QUERY_OK = 0
+SS_QUERY_REJECTED = 0xFFFFFFFF
+SS_QUERY_FAULT = 0x0BAD0001
 SS_NONE = 0xFFFFFFFF
 SS_MAX_L = 512
 SS_MAX_K = 512
@@ -44,7 +48,7 @@ class QueryStats(ctypes.Structure):
         ("ndc", ctypes.c_uint32),
         ("hops", ctypes.c_uint32),
         ("lossy", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("status", ctypes.c_uint32),
         ("t_start", ctypes.c_uint64),
         ("t_end", ctypes.c_uint64),
     ]

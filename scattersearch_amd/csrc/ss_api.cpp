@@ -768,6 +768,15 @@ static int search_host(ss_index *ix, const float *queries, uint32_t nq, const ui
         return hip_fail(e, "download of stats");
     if ((e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "search");
+    if (stats) {
+        for (uint32_t q = 0; q < nq; q++) {
+            if (stats[q].status == SS_QUERY_FAULT) {
+                set_error("search: internal error, query " + std::to_string(q) +
+                          " met a node id outside the index (result not valid)");
+                return SS_EINTERNAL;
+            }
+        }
+    }
     return SS_OK;
 }
 

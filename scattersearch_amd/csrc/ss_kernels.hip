@@ -441,7 +441,7 @@ __device__ __forceinline__ void st_stats(ss_query_stats *p, const ss_query_stats
         st_word<true>(w + 0, s.ndc);
         st_word<true>(w + 1, s.hops);
         st_word<true>(w + 2, s.lossy);
-        st_word<true>(w + 3, s.reserved);
+        st_word<true>(w + 3, s.status);
         st_word<true>(w + 4, (uint32_t)s.t_start);
         st_word<true>(w + 5, (uint32_t)(s.t_start >> 32));
         st_word<true>(w + 6, (uint32_t)s.t_end);
@@ -453,7 +453,7 @@ __device__ __forceinline__ void st_stats(ss_query_stats *p, const ss_query_stats
 
 // Per-query state shared by the search variants.
 struct QueryState {
-    uint32_t K, L, t, ndc, hops, lossy;
+    uint32_t K, L, t, ndc, hops, lossy, status;
     uint64_t t_start;
     uint32_t *out_ids;
     float *out_d;
@@ -472,6 +472,7 @@ __device__ __forceinline__ bool query_prologue(const SearchArgs &a, const WaveLd
     st.out_ids = a.out_ids + (size_t)q * a.k_stride;
     st.out_d = a.out_dists ? a.out_dists + (size_t)q * a.k_stride : nullptr;
     st.lossy = 0;
+    st.status = SS_QUERY_OK;
     st.hops = 0;
     st.ndc = 0;
     st.t = 0;
@@ -483,7 +484,7 @@ __device__ __forceinline__ bool query_prologue(const SearchArgs &a, const WaveLd
         }
         if (a.stats && lane == 0)
             st_stats<RING>(a.stats + q,
-                           ss_query_stats{0u, 0u, 0u, 0xFFFFFFFFu, st.t_start, (uint64_t)wall_clock64()});
+                           ss_query_stats{0u, 0u, 0u, SS_QUERY_REJECTED, st.t_start, (uint64_t)wall_clock64()});
         return false;
     }
     if constexpr (RING) {
@@ -572,8 +573,20 @@ __device__ __forceinline__ void query_end(const SearchArgs &a, const WaveLds &w,
     for (int off = 32; off > 0; off >>= 1)
         lsum += __shfl_xor(lsum, off, 64);
     if (a.stats && lane == 0)
-        st_stats<RING>(a.stats + q, ss_query_stats{st.ndc, st.hops, lsum, 0u, st.t_start, (uint64_t)wall_clock64()});
+        st_stats<RING>(a.stats + q, ss_query_stats{st.ndc, st.hops, lsum, st.status, st.t_start, (uint64_t)wall_clock64()});
     wave_sync();
+}
+
+// Adjacency row of node `id` (one id per lane, kNone past r_pad).  An id
+// outside the index would be an internal error; it is never dereferenced:
+// the query records SS_QUERY_FAULT and gets an empty row.
+__device__ __forceinline__ uint32_t load_row(const SearchArgs &a, uint32_t id, uint32_t lane, QueryState &st)
+{
+    if (id >= a.num) {
+        st.status = SS_QUERY_FAULT;
+        return kNone;
+    }
+    return lane < a.r_pad ? a.adj[(size_t)id * a.r_pad + lane] : kNone;
 }
 
 // Hook of expand_loop that never stops the search.
@@ -597,6 +610,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
                                             Hook &&hook)
 {
     constexpr int kMaxChunks = MC;  // pool chunks of 64 held in registers (L <= 64 * MC)
+    constexpr bool kRegPool = MC == 1;
     const uint32_t lane = lane_id();
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
@@ -627,12 +641,80 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     }
     uint32_t nb = kNone;
     if (have)
-        nb = lane < a.r_pad ? a.adj[(size_t)n0 * a.r_pad + lane] : kNone;
+        nb = load_row(a, n0, lane, st);
     while (have) {
         if (!hook(st, t))
             break;
-        const float tail = readlane_f(w.pool_d[t - 1], 0);
-        const uint32_t n = uniform(w.pool_id[k]) & ~kExplored;  // a thief may have marked it
+        // The pool of a step: a pool of at most 64 entries is read into
+        // registers once, here, and serves the whole step (tail, node, next
+        // entry, merge); larger pools are scanned here and read again for
+        // the merge (registers).  Entry k is marked explored (in the
+        // registers too).
+        const uint32_t nch = (t + 63u) >> 6;
+        uint32_t pid[kMaxChunks];
+        float pdv[kMaxChunks];
+        uint32_t cnt[kMaxChunks];
+        float tail;
+        uint32_t n = kNone;
+        // First unexplored entry after k (every entry up to k is explored):
+        // its row is the likely next one.  Id and distance come from the
+        // values the ballot tested.
+        int onext = -1;
+        uint32_t oid = kNone;
+        float od = __builtin_inff();
+        if constexpr (kRegPool) {
+            const uint32_t kc = k >> 6, kl = k & 63u, tc = (t - 1u) >> 6, tl = (t - 1u) & 63u;
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; c++) {
+                pid[c] = kNone;
+                pdv[c] = __builtin_inff();
+                const uint32_t i = (uint32_t)c * 64u + lane;
+                if ((uint32_t)c < nch && i < t) {
+                    pid[c] = w.pool_id[i];
+                    pdv[c] = w.pool_d[i];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; c++) {
+                if ((uint32_t)c == tc)
+                    tail = readlane_f(pdv[c], tl);
+                if ((uint32_t)c == kc) {
+                    n = readlane_u(pid[c], kl) & ~kExplored;  // a thief may have marked it
+                    if (lane == kl)
+                        pid[c] |= kExplored;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; c++) {
+                if ((uint32_t)c < nch && onext < 0) {
+                    const uint32_t i = (uint32_t)c * 64u + lane;
+                    const uint64_t m = ballot(i > k && i < t && !(pid[c] & kExplored));
+                    if (m) {
+                        const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        onext = c * 64 + (int)l;
+                        oid = readlane_u(pid[c], l);
+                        od = readlane_f(pdv[c], l);
+                    }
+                }
+            }
+        } else {
+            tail = readlane_f(w.pool_d[t - 1], 0);
+            n = uniform(w.pool_id[k]) & ~kExplored;  // a thief may have marked it
+            for (uint32_t c0 = (k + 1u) & ~63u; c0 < t; c0 += 64) {
+                const uint32_t i = c0 + lane;
+                const bool in = i > k && i < t;
+                const uint32_t v = in ? w.pool_id[i] : kNone;
+                const float dv = in ? w.pool_d[i] : __builtin_inff();
+                const uint64_t m = ballot(!(v & kExplored));
+                if (m) {
+                    const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    onext = (int)(c0 + l);
+                    oid = readlane_u(v, l);
+                    od = readlane_f(dv, l);
+                    break;
+                }
+            }
+        }
         wave_sync();
         if (lane == 0) {
             w.pool_id[k] = n | kExplored;
@@ -641,34 +723,13 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
         st.hops++;
 
-        // First unexplored entry after k (every entry up to k is explored);
-        // its row is the likely next one.  Id and distance come from the
-        // values the ballot tested.
-        int onext = -1;
-        uint32_t oid = kNone;
-        float od = __builtin_inff();
-        for (uint32_t c0 = (k + 1u) & ~63u; c0 < t; c0 += 64) {
-            const uint32_t i = c0 + lane;
-            const bool in = i > k && i < t;
-            const uint32_t v = in ? w.pool_id[i] : kNone;
-            const float dv = in ? w.pool_d[i] : __builtin_inff();
-            const uint64_t m = ballot(!(v & kExplored));
-            if (m) {
-                const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                onext = (int)(c0 + l);
-                oid = readlane_u(v, l);
-                od = readlane_f(dv, l);
-                break;
-            }
-        }
-
         // The row of this node is needed now; the likely next row is issued
         // only after it has arrived (vmcnt(0): this row is the only vector
         // memory access in flight), so that no later wait for this row also
         // waits for that one: the counter of loads in flight is in order.
         __builtin_amdgcn_s_waitcnt(0x0F70);
         __builtin_amdgcn_sched_barrier(0);
-        const uint32_t nbs = onext >= 0 && lane < a.r_pad ? a.adj[(size_t)oid * a.r_pad + lane] : kNone;
+        const uint32_t nbs = onext >= 0 ? load_row(a, oid, lane, st) : kNone;
 
         // Visited test of the whole row, compaction of the fresh ids.
         VisitClaim claim{kNone, kNone};
@@ -728,19 +789,17 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         wave_sync();
         bool inA = lane < nc && cd < tail;  // strict tail filter, :238
 
-        const uint32_t nch = (t + 63u) >> 6;
-        uint32_t pid[kMaxChunks];
-        float pdv[kMaxChunks];
-        uint32_t cnt[kMaxChunks];
 #pragma unroll
         for (int c = 0; c < kMaxChunks; c++) {
-            pid[c] = kNone;
-            pdv[c] = __builtin_inff();
             cnt[c] = 0;
-            const uint32_t i = (uint32_t)c * 64u + lane;
-            if ((uint32_t)c < nch && i < t) {
-                pid[c] = w.pool_id[i];
-                pdv[c] = w.pool_d[i];
+            if constexpr (!kRegPool) {
+                pid[c] = kNone;
+                pdv[c] = __builtin_inff();
+                const uint32_t i = (uint32_t)c * 64u + lane;
+                if ((uint32_t)c < nch && i < t) {
+                    pid[c] = w.pool_id[i];
+                    pdv[c] = w.pool_d[i];
+                }
             }
         }
 
@@ -771,41 +830,75 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
         const uint32_t na = (uint32_t)__popcll(am);
 
+        // The admitted candidates, compacted in row order, are staged in LDS
+        // as (distance, lane) and padded with (+inf, no lane) to a multiple
+        // of four; every lane reads them back with broadcast 16-byte reads,
+        // so the minimum and the merge counts below are plain vector
+        // arithmetic with no cross-lane reads in a dependent chain.
+        {
+            // Two independent writes: a lane can owe both a candidate (at its
+            // rank) and a padding slot (at its own index).
+            const uint32_t ra = lanes_below(am);
+            const uint32_t na4 = (na + 3u) & ~3u;
+            if (inA) {
+                w.cand_d[ra] = cd;
+                w.cand_id[ra] = lane;
+            }
+            if (lane >= na && lane < na4) {
+                w.cand_d[lane] = __builtin_inff();
+                w.cand_id[lane] = 64u;
+            }
+        }
+        wave_sync();
+        const float4 *sd4 = reinterpret_cast<const float4 *>(w.cand_d);
+        const uint4 *sj4 = reinterpret_cast<const uint4 *>(w.cand_id);
+
         // Next node: the first unexplored entry of the merged pool is either
         // the old one found above (it keeps position onext when no admitted
         // candidate is smaller, P first on ties) or the first smallest
         // admitted candidate; both lie inside the first L.
-        const float dmin = wave_min_f(inA ? cd : __builtin_inff());
+        float dmin = __builtin_inff();
+        for (uint32_t r0 = 0; r0 < na; r0 += 4) {
+            const float4 d4 = sd4[r0 >> 2];
+            dmin = d4.x < dmin ? d4.x : dmin;
+            dmin = d4.y < dmin ? d4.y : dmin;
+            dmin = d4.z < dmin ? d4.z : dmin;
+            dmin = d4.w < dmin ? d4.w : dmin;
+        }
         const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)ballot(inA && cd == dmin)) - 1u;
         const bool keep_old = onext >= 0 && od <= dmin;
         if (keep_old) {
             nb = nbs;
         } else {
-            const uint32_t next_id = readlane_u(cid, jmin);
-            nb = lane < a.r_pad ? a.adj[(size_t)next_id * a.r_pad + lane] : kNone;
+            nb = load_row(a, readlane_u(cid, jmin), lane, st);
         }
 
         // Merge ranks.  Pool entry i moves to i + #{a in A: a < p};
-        // candidate a goes to rankA(a) + #{p in P: p <= a}.  Branch-free
-        // body: one pass per admitted candidate.
+        // candidate a goes to rankA(a) + #{p in P: p <= a}, where rankA
+        // orders A by (distance, row position).
         uint32_t rankA = 0, posA = 0;
         const uint32_t inA_u = inA ? 1u : 0u;
-        uint64_t it = am;
-        while (it) {
-            const uint32_t j = (uint32_t)__ffsll((unsigned long long)it) - 1u;
-            it &= it - 1ull;
-            const float dj = readlane_f(cd, j);
-            rankA += inA_u & ((dj < cd ? 1u : 0u) | ((dj == cd ? 1u : 0u) & (j < lane ? 1u : 0u)));
-            uint32_t le = 0;
+        for (uint32_t r0 = 0; r0 < na; r0 += 4) {
+            const float4 d4 = sd4[r0 >> 2];
+            const uint4 j4 = sj4[r0 >> 2];
+            const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
+            const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
 #pragma unroll
-            for (int c = 0; c < kMaxChunks; c++) {
-                if ((uint32_t)c < nch) {
-                    cnt[c] += dj < pdv[c] ? 1u : 0u;
-                    // entries past t hold +inf and never count
-                    le += (uint32_t)__popcll(ballot(pdv[c] <= dj));
+            for (int e = 0; e < 4; e++) {
+                const float dj = dv[e];
+                const uint32_t jl = jv[e];
+                rankA += inA_u & ((dj < cd ? 1u : 0u) | ((dj == cd ? 1u : 0u) & (jl < lane ? 1u : 0u)));
+                uint32_t le = 0;
+#pragma unroll
+                for (int c = 0; c < kMaxChunks; c++) {
+                    if ((uint32_t)c < nch) {
+                        cnt[c] += dj < pdv[c] ? 1u : 0u;
+                        // entries past t hold +inf and never count
+                        le += (uint32_t)__popcll(ballot(pdv[c] <= dj));
+                    }
                 }
+                posA = lane == jl ? le : posA;  // padding has lane 64: nobody
             }
-            posA = lane == j ? le : posA;
         }
         posA += rankA;
         k = keep_old ? (uint32_t)onext : readlane_u(posA, jmin);  // :244-245
@@ -1032,14 +1125,13 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
     query_end(a, w, q, st);
 }
 
-// Minimum wavefronts per SIMD the register budget is sized for.  The 128-d
-// and runtime-size bodies fit in 128 VGPRs (four wavefronts per SIMD, 16 per
-// CU, which is also what the LDS of a small pool allows).  The compiler keeps
-// the lane's share of the query in registers across the neighbour loop; for
-// 96-d that needs up to 168 VGPRs and for 200-d up to 256 (whose visited
-// tables limit residency by LDS before registers do).
+// Minimum wavefronts per SIMD the register budget is sized for: up to 128-d
+// (and the runtime-size body) the search fits in 128 VGPRs, four wavefronts
+// per SIMD (16 per CU, which is also what the LDS of a small pool allows);
+// 200-d bodies take up to 256 (their visited tables limit residency by LDS
+// before registers do).
 template <int NBLK>
-constexpr int search_min_waves() { return (NBLK == 8 || NBLK < 0) ? 4 : (NBLK <= 6 ? 3 : 2); }
+constexpr int search_min_waves() { return NBLK <= 8 ? 4 : 2; }
 
 template <int NBLK, int TAILI, int METRIC, int EW, int MC>
 __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(SearchArgs a)
@@ -1529,8 +1621,14 @@ __device__ void scatter_query(const SearchArgs &a, const ScatterLds &s, uint32_t
 }
 
 // Blocks of W wavefronts, one query per block at a time.
+// Register budget of the several-wavefronts-per-query kernel (its control
+// and stealing state adds to the search body's): 96-d bodies get up to 168
+// VGPRs (three wavefronts per SIMD).
+template <int NBLK>
+constexpr int scatter_min_waves() { return (NBLK == 8 || NBLK < 0) ? 4 : (NBLK <= 6 ? 3 : 2); }
+
 template <int NBLK, int TAILI, int METRIC, int MC>
-__global__ __launch_bounds__(512, search_min_waves<NBLK>()) void scatter_kernel(SearchArgs a)
+__global__ __launch_bounds__(512, scatter_min_waves<NBLK>()) void scatter_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wave = threadIdx.x >> 6;

@@ -307,6 +307,7 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     a.search.dim_pad = s->v.dim_pad;
     a.search.r_pad = s->v.r_pad;
     a.search.ep = s->v.ep;
+    a.search.num = s->v.num;
     a.search.l_max = c.l_max;
     a.search.hash_slots = slots;
     uint32_t lg = 0;

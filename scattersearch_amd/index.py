@@ -15,7 +15,7 @@ METRIC_L2 = 0
 METRIC_INNER_PRODUCT = 1
 
 STATS_DTYPE = np.dtype([("ndc", np.uint32), ("hops", np.uint32),
-                        ("lossy", np.uint32), ("reserved", np.uint32),
+                        ("lossy", np.uint32), ("status", np.uint32),
                         ("t_start", np.uint64), ("t_end", np.uint64)])
 
 

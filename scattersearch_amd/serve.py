@@ -6,7 +6,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import QueryStats, ServeConfig, check, fptr, lib, uptr
+from . import _lib
+from ._lib import QueryStats, ServeConfig, SSError, check, fptr, lib, uptr
 from .index import STATS_DTYPE
 
 
@@ -48,6 +49,7 @@ class Server:
         check(lib.ss_poll(self._h, max_results, tk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), uptr(ids),
                           fptr(ds), st.ctypes.data_as(ctypes.POINTER(QueryStats)), ctypes.byref(n)))
         k = n.value
+        _check_status(st[:k])
         return tk[:k], ids[:k], ds[:k], st[:k]
 
     def stop(self):
@@ -87,7 +89,15 @@ def replay(index, queries, K, L, arrival_us, k_max=None, l_max=None, **kw):
     check(lib.ss_serve_replay(index.handle, ctypes.byref(cfg), fptr(q), nq, uptr(Ka), uptr(La),
                               arr.ctypes.data_as(dp), k_max, uptr(ids), fptr(ds), lat.ctypes.data_as(dp),
                               st.ctypes.data_as(ctypes.POINTER(QueryStats))))
+    _check_status(st)
     return ids, ds, lat, st
+
+
+def _check_status(st):
+    bad = np.nonzero(st["status"] == _lib.SS_QUERY_FAULT)[0]
+    if bad.size:
+        raise SSError(_lib.SS_EINTERNAL, f"serving: internal error, request {int(bad[0])} met a node id "
+                                         "outside the index (result not valid)")
 
 
 def poisson_arrivals(n, rate_qps, seed):
