@@ -216,18 +216,6 @@ __device__ __forceinline__ void team_dists(const float *const (&rows)[P], const 
     }
 }
 
-// Minimum of v over the wavefront (every lane active): DPP butterflies inside
-// each row of 16 lanes, then the four row minima.
-
-__device__ __forceinline__ float wave_min_f(float v)
-{
-    v = fminf(v, dpp_f<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = fminf(v, dpp_f<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = fminf(v, dpp_f<0x124>(v));  // row_ror:4
-    v = fminf(v, dpp_f<0x128>(v));  // row_ror:8
-    return fminf(fminf(readlane_f(v, 0), readlane_f(v, 16)), fminf(readlane_f(v, 32), readlane_f(v, 48)));
-}
-
 // Whole-vector distance in one lane (brute force).  q is the query, row the
 // base vector; same arithmetic as team_dists.
 template <int NBLK, int TAILI, int METRIC>
@@ -910,6 +898,8 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
 #pragma unroll
             for (int e = 0; e < 4; e++) {
+                if (r0 + (uint32_t)e >= na)  // padding: nothing to count
+                    break;
                 const float dj = dv[e];
                 const uint32_t jl = jv[e];
                 rankA += inA_u & ((dj < cd ? 1u : 0u) | ((dj == cd ? 1u : 0u) & (jl < lane ? 1u : 0u)));

@@ -65,3 +65,20 @@ def test_null_index_calls_fail_cleanly():
     assert lib.ss_destroy(None) == 0
     assert lib.ss_set_graph(None, None, 4, 0) == SS_EINVAL
     assert lib.ss_search(None, None, 1, None, None, 1, 1, 1, None, None, None, None) == SS_EINVAL
+
+
+def test_status_codes_match_the_header():
+    """The Python constants are the header's: error codes, per-query status
+    words, and the stats layout the kernels write (status is its 4th word)."""
+    import ctypes
+    import re
+    from scattersearch_amd import _lib
+    from scattersearch_amd.index import STATS_DTYPE
+    hdr = open(os.path.join(ROOT, "include", "scattersearch.h")).read()
+    defs = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"#define (SS_\w+) \(?(-?[0-9xA-Fa-f]+)u?\)?", hdr)}
+    for name in ("SS_OK", "SS_EINVAL", "SS_ENOMEM", "SS_EIO", "SS_EDEVICE", "SS_ESTATE", "SS_EBUSY",
+                 "SS_EINTERNAL", "SS_QUERY_OK", "SS_QUERY_REJECTED", "SS_QUERY_FAULT"):
+        assert name in defs, name
+        assert getattr(_lib, name) == defs[name], name
+    assert ctypes.sizeof(_lib.QueryStats) == 32 == STATS_DTYPE.itemsize
+    assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
