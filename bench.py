@@ -68,8 +68,8 @@ def main():
                     help="only build, then run --steps batched launches (for rocprofv3)")
     ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000",
                     help="Poisson arrival rates (QPS) of the serving sweep; empty to skip")
-    ap.add_argument("--batch-sizes", default="",
-                    help="also time hipGraph-replayed launches of these batch sizes (e.g. 1,64,1024,10000)")
+    ap.add_argument("--batch-sizes", default="1,64,1024,10000",
+                    help="also time hipGraph-replayed launches of these batch sizes; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=0.5,
                     help="length of the arrival stream at each serving rate (at least 20,000 requests)")
     args = ap.parse_args()
