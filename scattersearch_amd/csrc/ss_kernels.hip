@@ -270,18 +270,16 @@ __device__ __forceinline__ float lane_dist(const float *__restrict__ row, const 
 
 // ---------------------------------------------------------------------------
 // Visited set: a hash of node ids in LDS made of 32-byte buckets of eight
-// slots.  An id lives in its home bucket or, when that is full, in the next
-// one (cyclically), so a lookup reads both (four 16-byte LDS reads, one round
-// trip).  An absent id claims a free slot with a compare-and-swap, in the
-// home bucket first, starting at slot (id mod 8) so that two ids of one
-// bucket rarely race for the same slot.  The claim's outcome is checked
-// later (visit_settle), off the step's critical path: a claim that lost its
-// slot tries the two buckets' other slots.  An id that finds both buckets
-// full is treated as unvisited without being recorded (a lossy insert,
-// counted).  Forgetting a visited node never changes the search result: its
-// recomputed distance is either not below the pool tail (the tail only
-// decreases) or it is still in the pool, where the merge drops the repeat;
-// only the distance count grows.
+// slots.  A lookup is one read of the id's bucket (two 16-byte LDS reads);
+// an absent id claims a free slot of the bucket with a compare-and-swap,
+// starting at slot (id mod 8) so that two ids of one bucket rarely race for
+// the same slot.  The claim's outcome is checked later (visit_settle), off
+// the step's critical path: a claim that lost its slot tries the bucket's
+// other free slots.  A lane that finds its bucket full treats its id as
+// unvisited without recording it (a lossy insert, counted).  Forgetting a
+// visited node never changes the search result: its recomputed distance is
+// either not below the pool tail (the tail only decreases) or it is still in
+// the pool, where the merge drops the repeat; only the distance count grows.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBucket = 8;
 
@@ -290,64 +288,41 @@ struct VisitClaim {
     uint32_t old;   // what the compare-and-swap found there
 };
 
-__device__ __forceinline__ uint32_t free_mask(const uint4 x, const uint4 y)
-{
-    return (x.x == kNone ? 1u : 0u) | (x.y == kNone ? 2u : 0u) | (x.z == kNone ? 4u : 0u) |
-           (x.w == kNone ? 8u : 0u) | (y.x == kNone ? 16u : 0u) | (y.y == kNone ? 32u : 0u) |
-           (y.z == kNone ? 64u : 0u) | (y.w == kNone ? 128u : 0u);
-}
-
-__device__ __forceinline__ bool holds(const uint4 x, const uint4 y, uint32_t id)
-{
-    return x.x == id || x.y == id || x.z == id || x.w == id || y.x == id || y.y == id || y.z == id || y.w == id;
-}
-
 // bshift = 32 - log2(number of buckets).  Returns whether id was absent.
 __device__ __forceinline__ bool visit_begin(uint32_t *tab, uint32_t bshift, uint32_t id, VisitClaim &c,
                                             uint32_t &lossy)
 {
     c.slot = kNone;
     c.old = kNone;
-    const uint32_t nbk = 1u << (32u - bshift);
-    const uint32_t b0 = (id * 0x9E3779B1u) >> bshift;
-    const uint32_t b1 = (b0 + 1u) & (nbk - 1u);
-    const uint4 *p0 = reinterpret_cast<const uint4 *>(tab + b0 * kBucket);
-    const uint4 *p1 = reinterpret_cast<const uint4 *>(tab + b1 * kBucket);
-    const uint4 x0 = p0[0], y0 = p0[1], x1 = p1[0], y1 = p1[1];
-    if (holds(x0, y0, id) || holds(x1, y1, id))
+    const uint32_t b = ((id * 0x9E3779B1u) >> bshift) * kBucket;
+    const uint4 *t4 = reinterpret_cast<const uint4 *>(tab + b);
+    const uint4 x = t4[0], y = t4[1];
+    if (x.x == id || x.y == id || x.z == id || x.w == id || y.x == id || y.y == id || y.z == id || y.w == id)
         return false;
-    const uint32_t s0 = id & (kBucket - 1);
-    uint32_t fm = free_mask(x0, y0), base = b0 * kBucket;
-    if (fm == 0) {
-        fm = free_mask(x1, y1);
-        base = b1 * kBucket;
-    }
+    const uint32_t fm = (x.x == kNone ? 1u : 0u) | (x.y == kNone ? 2u : 0u) | (x.z == kNone ? 4u : 0u) |
+                        (x.w == kNone ? 8u : 0u) | (y.x == kNone ? 16u : 0u) | (y.y == kNone ? 32u : 0u) |
+                        (y.z == kNone ? 64u : 0u) | (y.w == kNone ? 128u : 0u);
     if (fm == 0) {
         lossy++;
         return true;
     }
+    const uint32_t s0 = id & (kBucket - 1);
     const uint32_t rot = ((fm >> s0) | (fm << (kBucket - s0))) & 0xFFu;
-    c.slot = base + ((s0 + (uint32_t)__builtin_ctz(rot)) & (kBucket - 1));
+    c.slot = b + ((s0 + (uint32_t)__builtin_ctz(rot)) & (kBucket - 1));
     c.old = atomicCAS(&tab[c.slot], kNone, id);
     return true;
 }
 
 // Completes a claim of visit_begin: a claim that found its slot taken (by
 // another lane of this step, or another wavefront of the query) tries the
-// other slots of the id's two buckets.
-__device__ __forceinline__ void visit_settle(uint32_t *tab, uint32_t bshift, uint32_t id, const VisitClaim &c,
-                                             uint32_t &lossy)
+// bucket's remaining slots.
+__device__ __forceinline__ void visit_settle(uint32_t *tab, uint32_t id, const VisitClaim &c, uint32_t &lossy)
 {
     if (c.slot == kNone || c.old == kNone || c.old == id)
         return;
-    const uint32_t nbk = 1u << (32u - bshift);
-    const uint32_t b0 = (id * 0x9E3779B1u) >> bshift;
-    const uint32_t b1 = (b0 + 1u) & (nbk - 1u);
-#pragma unroll 1
-    for (uint32_t i = 0; i < 2 * kBucket; i++) {  // rare: not unrolled
-        const uint32_t s = (i < kBucket ? b0 : b1) * kBucket + (i & (kBucket - 1));
-        if (s == c.slot)
-            continue;
+    const uint32_t b = c.slot & ~(kBucket - 1);
+    for (uint32_t i = 1; i < kBucket; i++) {
+        const uint32_t s = b + ((c.slot + i) & (kBucket - 1));
         const uint32_t o = atomicCAS(&tab[s], kNone, id);
         if (o == kNone || o == id)
             return;
@@ -360,7 +335,7 @@ __device__ __forceinline__ bool visit(uint32_t *tab, uint32_t bshift, uint32_t i
 {
     VisitClaim c;
     const bool fresh = visit_begin(tab, bshift, id, c, lossy);
-    visit_settle(tab, bshift, id, c, lossy);
+    visit_settle(tab, id, c, lossy);
     return fresh;
 }
 
@@ -798,7 +773,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
         st.ndc += nc;
         if (fresh)
-            visit_settle(w.hash, bshift, nb, claim, st.lossy);
+            visit_settle(w.hash, nb, claim, st.lossy);
         wave_sync();
         bool inA = lane < nc && cd < tail;  // strict tail filter, :238
 
