@@ -150,7 +150,13 @@ int ss_search_path(ss_index *index, const float *queries, uint32_t nq, uint32_t 
 /* Same with every buffer already on the device: d_queries is
  * [nq][dim_pad]; k_max / l_max bound K[q] / L[q] (they size on-chip state);
  * stream is a hipStream_t (NULL = default stream).  Asynchronous: returns
- * after enqueueing. */
+ * after enqueueing.
+ *
+ * Concurrency: an index holds one work-queue counter, one cached hipGraph and
+ * the host-side staging buffers of ss_search, so searches on one index must
+ * not overlap: call it from one thread at a time, and enqueue device searches
+ * on one stream at a time (or order the streams).  A batch already runs all
+ * of its queries concurrently on the device. */
 int ss_search_device(ss_index *index, const float *d_queries, uint32_t nq,
                      const uint32_t *d_K, const uint32_t *d_L, uint32_t k_max, uint32_t l_max,
                      uint32_t k_stride, const ss_search_params *params,
