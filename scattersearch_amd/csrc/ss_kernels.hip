@@ -52,7 +52,25 @@ __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 
+// Unsigned key with the order and equality of the distance d (not NaN):
+// squared L2 distances are +0 or positive, never -0, so their bit patterns
+// already order like the values; negated inner products can be -0, which is
+// made +0 before the sign-magnitude to offset-binary map.
+template <int METRIC>
+__device__ __forceinline__ uint32_t dist_key(float d)
+{
+    if constexpr (METRIC == SS_METRIC_L2) {
+        return __float_as_uint(d);
+    } else {
+        const uint32_t b = __float_as_uint(d + 0.0f);
+        return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    }
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Bit lane_id() of the wave-uniform mask m, as a per-lane condition.
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 {
@@ -297,18 +315,24 @@ __device__ __forceinline__ bool visit_begin(uint32_t *tab, uint32_t bshift, uint
     const uint32_t b = ((id * 0x9E3779B1u) >> bshift) * kBucket;
     const uint4 *t4 = reinterpret_cast<const uint4 *>(tab + b);
     const uint4 x = t4[0], y = t4[1];
-    if (x.x == id || x.y == id || x.z == id || x.w == id || y.x == id || y.y == id || y.z == id || y.w == id)
+    // Present: the smallest XOR of a slot with id is zero (three-input
+    // minimums, no per-slot compare and select).
+    const uint32_t z = min(min(min(min(x.x ^ id, x.y ^ id), x.z ^ id), min(min(x.w ^ id, y.x ^ id), y.y ^ id)),
+                           min(y.z ^ id, y.w ^ id));
+    if (z == 0)
         return false;
-    const uint32_t fm = (x.x == kNone ? 1u : 0u) | (x.y == kNone ? 2u : 0u) | (x.z == kNone ? 4u : 0u) |
-                        (x.w == kNone ? 8u : 0u) | (y.x == kNone ? 16u : 0u) | (y.y == kNone ? 32u : 0u) |
-                        (y.z == kNone ? 64u : 0u) | (y.w == kNone ? 128u : 0u);
+    // Free slots hold kNone, the only value with bit 31 set (ids are below
+    // 2^31 - 1): bit s of fm = bit 31 of slot s.
+    const uint32_t fm = (x.x >> 31) | ((x.y >> 30) & 2u) | ((x.z >> 29) & 4u) | ((x.w >> 28) & 8u) |
+                        ((y.x >> 27) & 16u) | ((y.y >> 26) & 32u) | ((y.z >> 25) & 64u) | ((y.w >> 24) & 128u);
     if (fm == 0) {
         lossy++;
         return true;
     }
+    // First free slot at or after slot (id mod 8), cyclically.
     const uint32_t s0 = id & (kBucket - 1);
-    const uint32_t rot = ((fm >> s0) | (fm << (kBucket - s0))) & 0xFFu;
-    c.slot = b + ((s0 + (uint32_t)__builtin_ctz(rot)) & (kBucket - 1));
+    const uint32_t f2 = fm | (fm << kBucket);
+    c.slot = b + ((s0 + (uint32_t)__builtin_ctz(f2 >> s0)) & (kBucket - 1));
     c.old = atomicCAS(&tab[c.slot], kNone, id);
     return true;
 }
@@ -818,43 +842,28 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
         const uint32_t na = (uint32_t)__popcll(am);
 
-        // The admitted candidates, compacted in row order, are staged in LDS
-        // as (distance, lane) and padded with (+inf, no lane) to a multiple
-        // of four; every lane reads them back with broadcast 16-byte reads,
-        // so the minimum and the merge counts below are plain vector
-        // arithmetic with no cross-lane reads in a dependent chain.
-        {
-            // Two independent writes: a lane can owe both a candidate (at its
-            // rank) and a padding slot (at its own index).
-            const uint32_t ra = lanes_below(am);
-            const uint32_t na4 = (na + 3u) & ~3u;
-            if (inA) {
-                w.cand_d[ra] = cd;
-                w.cand_id[ra] = lane;
-            }
-            if (lane >= na && lane < na4) {
-                w.cand_d[lane] = __builtin_inff();
-                w.cand_id[lane] = 64u;
-            }
-        }
-        wave_sync();
-        const float4 *sd4 = reinterpret_cast<const float4 *>(w.cand_d);
-        const uint4 *sj4 = reinterpret_cast<const uint4 *>(w.cand_id);
-
         // Next node: the first unexplored entry of the merged pool is either
         // the old one found above (it keeps position onext when no admitted
         // candidate is smaller, P first on ties) or the first smallest
-        // admitted candidate; both lie inside the first L.
-        float dmin = __builtin_inff();
-        for (uint32_t r0 = 0; r0 < na; r0 += 4) {
-            const float4 d4 = sd4[r0 >> 2];
-            dmin = d4.x < dmin ? d4.x : dmin;
-            dmin = d4.y < dmin ? d4.y : dmin;
-            dmin = d4.z < dmin ? d4.z : dmin;
-            dmin = d4.w < dmin ? d4.w : dmin;
+        // admitted candidate; both lie inside the first L.  The admitted
+        // lanes are walked in row order with scalar registers: one
+        // cross-lane read per candidate, the minimum kept by integer
+        // compares of order-preserving keys (a strict decrease keeps the
+        // first of equal distances).
+        uint32_t jmin = 64u;
+        {
+            const uint32_t key = dist_key<METRIC>(cd);
+            uint32_t best = kNone;
+            for (uint64_t rem = am; rem; rem &= rem - 1ull) {
+                const uint32_t j = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+                const uint32_t kj = readlane_u(key, j);
+                if (kj < best) {
+                    best = kj;
+                    jmin = j;
+                }
+            }
         }
-        const uint32_t jmin = (uint32_t)__ffsll((unsigned long long)ballot(inA && cd == dmin)) - 1u;
-        const bool keep_old = onext >= 0 && od <= dmin;
+        const bool keep_old = onext >= 0 && od <= readlane_f(cd, jmin);
         if (keep_old) {
             nb = nbs;
         } else {
@@ -863,32 +872,29 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
 
         // Merge ranks.  Pool entry i moves to i + #{a in A: a < p};
         // candidate a goes to rankA(a) + #{p in P: p <= a}, where rankA
-        // orders A by (distance, row position).
+        // orders A by (distance, row position).  One compare of candidate
+        // j's distance against the pool serves both counts: lanes where it
+        // is below the entry shift that entry, the others (entries past t
+        // hold +inf and always shift) are the entries at or below it.
+        // rankA is meaningful on admitted lanes only.
         uint32_t rankA = 0, posA = 0;
-        const uint32_t inA_u = inA ? 1u : 0u;
-        for (uint32_t r0 = 0; r0 < na; r0 += 4) {
-            const float4 d4 = sd4[r0 >> 2];
-            const uint4 j4 = sj4[r0 >> 2];
-            const float dv[4] = {d4.x, d4.y, d4.z, d4.w};
-            const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
+        for (uint64_t rem = am; rem; rem &= rem - 1ull) {
+            const uint32_t j = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+            const float dj = readlane_f(cd, j);
+            // j ranks before the lanes where it is smaller, and before the
+            // later lanes (row order) where it is equal.
+            const uint64_t after = (~0ull << j) << 1;
+            rankA += lane_bit(ballot(dj < cd) | (ballot(dj == cd) & after)) ? 1u : 0u;
+            uint32_t le = 0;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if (r0 + (uint32_t)e >= na)  // padding: nothing to count
-                    break;
-                const float dj = dv[e];
-                const uint32_t jl = jv[e];
-                rankA += inA_u & ((dj < cd ? 1u : 0u) | ((dj == cd ? 1u : 0u) & (jl < lane ? 1u : 0u)));
-                uint32_t le = 0;
-#pragma unroll
-                for (int c = 0; c < kMaxChunks; c++) {
-                    if ((uint32_t)c < nch) {
-                        cnt[c] += dj < pdv[c] ? 1u : 0u;
-                        // entries past t hold +inf and never count
-                        le += (uint32_t)__popcll(ballot(pdv[c] <= dj));
-                    }
+            for (int c = 0; c < kMaxChunks; c++) {
+                if (c == 0 || (uint32_t)c < nch) {  // t >= 1: chunk 0 always holds entries
+                    const bool below = dj < pdv[c];
+                    cnt[c] += below ? 1u : 0u;
+                    le += 64u - (uint32_t)__popcll(ballot(below));
                 }
-                posA = lane == jl ? le : posA;  // padding has lane 64: nobody
             }
+            posA = lane_bit(1ull << j) ? le : posA;
         }
         posA += rankA;
         k = keep_old ? (uint32_t)onext : readlane_u(posA, jmin);  // :244-245
