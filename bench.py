@@ -62,6 +62,8 @@ def main():
                     help="skip the wider-expansion and several-wavefronts-per-query measurements")
     ap.add_argument("--visited-slots", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--query-order", choices=["auto", "index", "locality"], default="auto",
+                    help="order in which the batched search takes the queries (results do not depend on it)")
     ap.add_argument("--graph-cache", default="",
                     help="NSG file: load the graph from it if it exists, else build and write it")
     ap.add_argument("--profile", action="store_true",
@@ -120,10 +122,19 @@ def main():
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()  # default priority; hipGraph capture needs a non-null stream
 
+    qorder = {"auto": ss._lib.SS_ORDER_AUTO, "index": ss._lib.SS_ORDER_INDEX,
+              "locality": ss._lib.SS_ORDER_LOCALITY}[args.query_order]
+
+    def order_used(n, wq=1):
+        if qorder == ss._lib.SS_ORDER_AUTO:
+            return "locality" if n >= ss._lib.SS_ORDER_MIN_BATCH and wq <= 1 else "index"
+        return args.query_order
+
     def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block, ew=0, wq=0):
         ix.search_device(dq.data_ptr(), nq, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
                          d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=use_graph,
-                         visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq)
+                         visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq,
+                         query_order=qorder)
 
     def stats_host():
         st = d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
@@ -247,7 +258,8 @@ def main():
 
         def launch_b(n=bsz):
             ix.search_device(dq.data_ptr(), n, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
-                             d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=True)
+                             d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=True,
+                             query_order=qorder)
 
         reps = max(3, min(200, 20000 // bsz))
         launch_b()
@@ -263,7 +275,7 @@ def main():
         kms = [a_ev.elapsed_time(b_ev) for a_ev, b_ev in evs]
         row = dict(batch=bsz, launches=reps, qps=round(bsz * reps / wall_b, 1),
                    launch_ms_p50=round(float(np.percentile(kms, 50)), 4),
-                   launch_ms_p99=round(float(np.percentile(kms, 99)), 4))
+                   launch_ms_p99=round(float(np.percentile(kms, 99)), 4), query_order=order_used(bsz))
         batches.append(row)
         log(f"batch {row}")
 
@@ -318,7 +330,8 @@ def main():
         "config": {"workload": w.name, "num": num, "dim": w.dim, "nq": nq, "R": w.R, "K": K, "L": L,
                    "metric": "L2" if w.metric == 0 else "inner_product", "recall@10": round(rec, 4),
                    "mean_ndc": round(float(st["ndc"].mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1),
-                   "launch": {k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots")}},
+                   "launch": dict({k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots")},
+                                  query_order=order_used(nq))},
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
         "serving": serving,

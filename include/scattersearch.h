@@ -69,8 +69,21 @@ typedef struct ss_search_params {
                                  wavefronts per query sharing the visited set,
                                  election at hop 50, stealing; recall instead of
                                  exact parity; expand_width must be 1 */
-    uint32_t reserved;
+    uint32_t query_order;     /* order in which the batched search takes the
+                                 queries (results do not depend on it):
+                                 SS_ORDER_AUTO (0): SS_ORDER_LOCALITY for
+                                 batches of at least SS_ORDER_MIN_BATCH queries
+                                 with one wavefront per query, else index
+                                 order; SS_ORDER_INDEX (1); SS_ORDER_LOCALITY
+                                 (2): queries near one another in space are
+                                 taken together, so that their searches share
+                                 cached vectors and rows */
 } ss_search_params;
+
+#define SS_ORDER_AUTO 0u
+#define SS_ORDER_INDEX 1u
+#define SS_ORDER_LOCALITY 2u
+#define SS_ORDER_MIN_BATCH 4096u
 
 #define SS_QUERY_OK 0u
 #define SS_QUERY_REJECTED 0xFFFFFFFFu

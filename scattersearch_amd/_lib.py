@@ -29,6 +29,11 @@ SS_MAX_RPAD = 64
 SS_MODE_BATCHED = 0
 SS_MODE_PER_QUERY = 1
 
+SS_ORDER_AUTO = 0
+SS_ORDER_INDEX = 1
+SS_ORDER_LOCALITY = 2
+SS_ORDER_MIN_BATCH = 4096
+
 
 class SearchParams(ctypes.Structure):
     _fields_ = [
@@ -39,7 +44,7 @@ class SearchParams(ctypes.Structure):
         ("use_graph", ctypes.c_uint32),
         ("expand_width", ctypes.c_uint32),
         ("waves_per_query", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("query_order", ctypes.c_uint32),
     ]
 
 

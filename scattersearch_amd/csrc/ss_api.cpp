@@ -78,6 +78,12 @@ struct ss_index {
     uint32_t r_pad = 0, ep = 0;
     bool has_graph = false;
     uint32_t *d_queue = nullptr;
+    // Locality order of batched searches: pivots [pivot_dims][kOrderPivots]
+    // (made on first use), keys and order of up to order_n queries.
+    float *d_pivots = nullptr;
+    uint32_t pivot_dims = 0;
+    DevBuf<uint32_t> order;
+    uint32_t order_n = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;  // serialises the host-buffer entry points
     DevBuf<float> q;
@@ -247,6 +253,9 @@ int ss_destroy(ss_index *ix)
         (void)hipFree(ix->d_adj);
     if (ix->d_queue)
         (void)hipFree(ix->d_queue);
+    if (ix->d_pivots)
+        (void)hipFree(ix->d_pivots);
+    ix->order.release();
     ix->q.release();
     ix->kl.release();
     ix->ids.release();
@@ -555,6 +564,35 @@ static uint32_t log2u(uint32_t x)
     return r;
 }
 
+// Pivots (once per index) and room for the keys and order of nq queries.
+static int prepare_order(ss_index *ix, uint32_t nq, hipStream_t st)
+{
+    hipError_t e;
+    if (!ix->d_pivots) {
+        const uint32_t dims = std::min<uint32_t>(ix->dim_pad, ss::kOrderKeyDims);
+        float *piv = nullptr;
+        if ((e = hipMalloc(reinterpret_cast<void **>(&piv), (size_t)dims * ss::kOrderPivots * sizeof(float))) !=
+            hipSuccess)
+            return hip_fail(e, "order pivots");
+        if ((e = ss::launch_order_pivots(ix->d_base, ix->dim_pad, ix->num, dims, piv, st)) != hipSuccess) {
+            (void)hipFree(piv);
+            return hip_fail(e, "order pivots");
+        }
+        ix->d_pivots = piv;
+        ix->pivot_dims = dims;
+    }
+    if (nq > ix->order_n) {
+        // A launch in flight may still read the old buffer.
+        if (ix->order.ptr && (e = hipDeviceSynchronize()) != hipSuccess)
+            return hip_fail(e, "order buffer");
+        ix->order_n = 0;
+        if ((e = ix->order.reserve(2 * (size_t)nq)) != hipSuccess)
+            return hip_fail(e, "order buffer");
+        ix->order_n = nq;
+    }
+    return SS_OK;
+}
+
 static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
                       uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
                       uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st,
@@ -577,6 +615,10 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     const uint32_t mode = p ? p->mode : SS_MODE_BATCHED;
     if (mode != SS_MODE_BATCHED && mode != SS_MODE_PER_QUERY) {
         set_error("search: unknown mode");
+        return SS_EINVAL;
+    }
+    if (p && p->query_order > SS_ORDER_LOCALITY) {
+        set_error("search: query_order must be SS_ORDER_AUTO, SS_ORDER_INDEX or SS_ORDER_LOCALITY");
         return SS_EINVAL;
     }
     const ss_search_params resolved = resolve_params(ix, l_max, p);
@@ -627,6 +669,27 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         return SS_OK;
     }
     a.queue = d_queue ? d_queue : ix->d_queue;
+    const bool locality = !d_path && !d_queue && a.waves_per_query == 1 &&
+                          (p->query_order == SS_ORDER_LOCALITY ||
+                           (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH));
+    ss::OrderArgs oa;
+    std::memset(&oa, 0, sizeof(oa));
+    if (locality) {
+        if ((rc = prepare_order(ix, nq, st)) != SS_OK)
+            return rc;
+        oa.queries = d_queries;
+        oa.pivots = ix->d_pivots;
+        oa.nq = nq;
+        oa.dim_pad = ix->dim_pad;
+        oa.key_dims = ix->pivot_dims;
+        oa.keys = ix->order.ptr;
+        oa.order = ix->order.ptr + ix->order_n;
+        oa.queue = a.queue;
+        a.order = oa.order;
+    }
+    auto reset = [&]() {
+        return locality ? ss::launch_order(oa, (uint32_t)ix->num_cus, st) : ss::launch_queue_reset(a.queue, st);
+    };
     const bool use_graph = p && p->use_graph && !d_path && !d_queue;
     if (use_graph) {
         const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
@@ -642,7 +705,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             }
             if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return hip_fail(e, "hipStreamBeginCapture");
-            hipError_t e1 = ss::launch_queue_reset(ix->d_queue, st);
+            hipError_t e1 = reset();
             hipError_t e2 = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st);
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(st, &g);
@@ -667,7 +730,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             return hip_fail(e, "hipGraphLaunch");
         return SS_OK;
     }
-    if ((e = ss::launch_queue_reset(a.queue, st)) != hipSuccess)
+    if ((e = reset()) != hipSuccess)
         return hip_fail(e, "queue reset");
     if ((e = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st)) != hipSuccess)
         return hip_fail(e, "search launch");

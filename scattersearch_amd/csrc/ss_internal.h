@@ -40,6 +40,24 @@ struct SearchArgs {
     uint32_t expand_width; // 1 (exact), 2 or 4 pool entries expanded per step
     uint32_t waves_per_query; // 1, or 2..8 wavefronts per query (ScatterSearch, blocks of that many)
     uint32_t num;          // base points (random entry points of the multi-wavefront search)
+    const uint32_t *order; // non-null: the work queue's position i holds query order[i]
+};
+
+// Locality order of a batched search (scheduling only).  Queries are keyed by
+// their nearest of kOrderPivots pivots (base points, first key_dims
+// coordinates) and counting-sorted by key, so that queries near one another
+// are taken from the work queue together and their searches, running at the
+// same time, share vectors and rows in the L2 and the MALL.
+constexpr uint32_t kOrderPivots = 64;
+constexpr uint32_t kOrderKeyDims = 32;
+
+struct OrderArgs {
+    const float *queries;  // [nq][dim_pad]
+    const float *pivots;   // [key_dims][kOrderPivots]
+    uint32_t nq, dim_pad, key_dims;
+    uint32_t *keys;        // [nq] scratch
+    uint32_t *order;       // [nq] out: query of each sorted position
+    uint32_t *queue;       // work-queue counter, cleared here
 };
 
 // A serving activation: wavefronts claim published requests from a ring in
@@ -116,6 +134,12 @@ hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
 hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_t waves_per_block,
                         hipStream_t stream);
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream);
+// pivots[i * kOrderPivots + p] = coordinate i < key_dims of base point
+// ((2p + 1) * num) / (2 kOrderPivots).
+hipError_t launch_order_pivots(const float *base, uint32_t dim_pad, uint32_t num, uint32_t key_dims, float *pivots,
+                               hipStream_t stream);
+// Keys, counting sort and work-queue reset of a locality-ordered launch.
+hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream);
 // what: 0 = random initial lists, 1 = one pull iteration
 hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, hipStream_t stream);
 hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream);

@@ -1142,10 +1142,15 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
     while (true) {
         uint32_t q;
         if (a.queue) {
-            q = 0;
+            uint32_t pos = 0;
             if (lane_id() == 0)
-                q = atomicAdd(a.queue, 1u);
-            q = uniform(q);
+                pos = atomicAdd(a.queue, 1u);
+            pos = uniform(pos);
+            if (pos >= a.nq)
+                break;
+            q = a.order ? uniform(a.order[pos]) : pos;
+            if (q >= a.nq)  // a malformed order entry: skip the position
+                continue;
         } else {
             if (!first)
                 break;
@@ -1162,6 +1167,131 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
 }
 
 __global__ void queue_reset_kernel(uint32_t *queue) { queue[0] = 0u; }
+
+// ---------------------------------------------------------------------------
+// Locality order of a batch.  Only the order in which wavefronts take the
+// queries changes: every search and its result are independent of it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void order_pivot_kernel(const float *base, uint32_t dim_pad, uint32_t num,
+                                                          uint32_t key_dims, float *piv)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= key_dims * kOrderPivots)
+        return;
+    const uint32_t i = t / kOrderPivots, p = t % kOrderPivots;
+    const uint32_t row = (uint32_t)(((uint64_t)(2u * p + 1u) * num) / (2u * kOrderPivots));
+    piv[t] = base[(size_t)row * dim_pad + i];
+}
+
+// One lane per pivot: lane p holds the first key_dims coordinates of pivot
+// p in registers (read once per wavefront, 256-byte coalesced rows of the
+// transposed pivots) and a wavefront keys one query after another, the next
+// query's coordinates (lane i holds coordinate i) read while the current one
+// is keyed.  Key = the nearest pivot in squared distance (lowest index on
+// ties).  Block 0 also clears the work-queue counter.
+constexpr uint32_t kOrderKeyWaves = 4;
+
+__global__ __launch_bounds__(64 * kOrderKeyWaves) void order_key_kernel(OrderArgs a)
+{
+    static_assert(kOrderKeyDims <= 64, "one query coordinate per lane");
+    const uint32_t lane = threadIdx.x & 63u;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        a.queue[0] = 0u;
+    const uint32_t stride = gridDim.x * kOrderKeyWaves;
+    uint32_t q = blockIdx.x * kOrderKeyWaves + (threadIdx.x >> 6);
+    if (q >= a.nq)
+        return;
+    float pv[kOrderKeyDims];
+#pragma unroll
+    for (uint32_t i = 0; i < kOrderKeyDims; i++)
+        pv[i] = i < a.key_dims ? a.pivots[i * kOrderPivots + lane] : 0.f;
+    float qc = lane < a.key_dims ? a.queries[(size_t)q * a.dim_pad + lane] : 0.f;
+    for (; q < a.nq; q += stride) {
+        const uint32_t qn = q + stride;
+        const float qnext = qn < a.nq && lane < a.key_dims ? a.queries[(size_t)qn * a.dim_pad + lane] : 0.f;
+        // Coordinates past key_dims are 0 in both the query and the pivots.
+        float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+        for (uint32_t i = 0; i < kOrderKeyDims; i += 2) {
+            const float d0 = readlane_f(qc, i) - pv[i];
+            const float d1 = readlane_f(qc, i + 1u) - pv[i + 1u];
+            acc0 = __builtin_fmaf(d0, d0, acc0);
+            acc1 = __builtin_fmaf(d1, d1, acc1);
+        }
+        const float acc = acc0 + acc1;
+        float m = acc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            m = fminf(m, __shfl_xor(m, off, 64));
+        const uint64_t hit = ballot(acc == m);  // empty only if every distance is NaN
+        const uint32_t key = hit ? (uint32_t)__ffsll((unsigned long long)hit) - 1u : 0u;
+        if (lane == 0)
+            a.keys[q] = key;
+        qc = qnext;
+    }
+}
+
+// Counting sort of the keys by one workgroup: histogram, exclusive scan (one
+// wavefront), scatter (positions within one key in any order).  Batches of
+// up to kOrderSortThreads * kOrderSortUnroll queries keep their keys in
+// registers between the two passes.
+constexpr uint32_t kOrderSortThreads = 1024;
+constexpr uint32_t kOrderSortUnroll = 16;
+
+__global__ __launch_bounds__(kOrderSortThreads) void order_sort_kernel(OrderArgs a)
+{
+    __shared__ uint32_t cnt[kOrderPivots];
+    __shared__ uint32_t off[kOrderPivots];
+    const uint32_t t = threadIdx.x;
+    constexpr uint32_t kStep = kOrderSortThreads * kOrderSortUnroll;
+    const bool one_pass = a.nq <= kStep;
+    if (t < kOrderPivots)
+        cnt[t] = 0u;
+    __syncthreads();
+    uint32_t k[kOrderSortUnroll];
+    for (uint32_t q0 = 0; q0 < a.nq; q0 += kStep) {
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderSortUnroll; u++) {
+            const uint32_t q = q0 + u * kOrderSortThreads + t;
+            k[u] = q < a.nq ? min(a.keys[q], kOrderPivots - 1u) : kNone;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderSortUnroll; u++)
+            if (k[u] != kNone)
+                atomicAdd(&cnt[k[u]], 1u);
+    }
+    __syncthreads();
+    static_assert(kOrderPivots == 64, "the scan gives one lane of a wavefront to each key");
+    if (t < 64) {
+        const uint32_t c = cnt[t];
+        uint32_t incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+            if ((int)t >= d)
+                incl += y;
+        }
+        off[t] = incl - c;
+    }
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < a.nq; q0 += kStep) {
+        if (!one_pass) {
+#pragma unroll
+            for (uint32_t u = 0; u < kOrderSortUnroll; u++) {
+                const uint32_t q = q0 + u * kOrderSortThreads + t;
+                k[u] = q < a.nq ? min(a.keys[q], kOrderPivots - 1u) : kNone;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderSortUnroll; u++) {
+            if (k[u] != kNone) {
+                const uint32_t pos = atomicAdd(&off[k[u]], 1u);
+                if (pos < a.nq)
+                    a.order[pos] = q0 + u * kOrderSortThreads + t;
+            }
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Several wavefronts per query: ScatterSearch
@@ -2400,6 +2530,32 @@ hipError_t launch_serve(int metric, const ServeArgs &a, uint32_t blocks, uint32_
 hipError_t launch_queue_reset(uint32_t *queue, hipStream_t stream)
 {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_pivots(const float *base, uint32_t dim_pad, uint32_t num, uint32_t key_dims, float *pivots,
+                               hipStream_t stream)
+{
+    if (key_dims == 0 || key_dims > kOrderKeyDims || key_dims > dim_pad || (key_dims & 1u) || num == 0)
+        return hipErrorInvalidValue;
+    const uint32_t n = key_dims * kOrderPivots;
+    hipLaunchKernelGGL(order_pivot_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, base, dim_pad, num,
+                       key_dims, pivots);
+    return hipGetLastError();
+}
+
+hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream)
+{
+    if (a.key_dims == 0 || a.key_dims > kOrderKeyDims || a.key_dims > a.dim_pad || (a.key_dims & 1u) || a.nq == 0)
+        return hipErrorInvalidValue;
+    const uint32_t per_block = 2 * kOrderKeyWaves;  // queries per workgroup and pass
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((a.nq + per_block - 1) / per_block,
+                                                                     std::max<uint32_t>(num_cus, 1) * 4));
+    hipLaunchKernelGGL(order_key_kernel, dim3(blocks), dim3(64 * kOrderKeyWaves), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(order_sort_kernel, dim3(1), dim3(kOrderSortThreads), 0, stream, a);
     return hipGetLastError();
 }
 

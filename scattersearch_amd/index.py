@@ -24,8 +24,9 @@ def dim_padded(dim):
 
 
 def _params(mode=_lib.SS_MODE_BATCHED, waves_per_block=0, visited_slots=0,
-            blocks=0, use_graph=False, expand_width=0, waves_per_query=0):
+            blocks=0, use_graph=False, expand_width=0, waves_per_query=0, query_order=0):
     p = SearchParams()
+    p.query_order = query_order
     p.expand_width = expand_width
     p.waves_per_query = waves_per_query
     p.mode = mode
@@ -113,9 +114,10 @@ class Index:
     # -- search -----------------------------------------------------------
     def search(self, queries, K, L, k_stride=None, mode=_lib.SS_MODE_BATCHED,
                waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0,
-               waves_per_query=0):
+               waves_per_query=0, query_order=0):
         """Search a batch of host queries.  K and L are ints or per-query
-        arrays.  Returns (ids [nq][k_stride], dists, stats)."""
+        arrays.  query_order is SS_ORDER_* (scheduling only; results do not
+        depend on it).  Returns (ids [nq][k_stride], dists, stats)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q[None, :]
@@ -137,7 +139,8 @@ class Index:
         ids = np.empty((nq, ks), dtype=np.uint32)
         dists = np.empty((nq, ks), dtype=np.float32)
         stats = np.zeros(nq, dtype=STATS_DTYPE)
-        p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query)
+        p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query,
+                    query_order)
         check(lib.ss_search(self._h, fptr(q), nq, uptr(Ka), uptr(La), k_def, l_def, ks,
                             ctypes.byref(p), uptr(ids), fptr(dists),
                             stats.ctypes.data_as(ctypes.POINTER(QueryStats))))
@@ -161,10 +164,11 @@ class Index:
     def search_device(self, d_queries, nq, k_max, l_max, k_stride, d_out_ids, d_out_dists,
                       d_stats=0, d_K=0, d_L=0, stream=0, mode=_lib.SS_MODE_BATCHED,
                       waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0,
-                      waves_per_query=0):
+                      waves_per_query=0, query_order=0):
         """Enqueue a search on device buffers (raw pointers, e.g. from torch
         tensors' data_ptr()); queries are [nq][dim_pad]."""
-        p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query)
+        p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query,
+                    query_order)
         vp = ctypes.c_void_p
         check(lib.ss_search_device(self._h, vp(d_queries), nq, vp(d_K or None), vp(d_L or None),
                                    k_max, l_max, k_stride, ctypes.byref(p), vp(d_out_ids),

@@ -77,8 +77,11 @@ def test_status_codes_match_the_header():
     hdr = open(os.path.join(ROOT, "include", "scattersearch.h")).read()
     defs = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"#define (SS_\w+) \(?(-?[0-9xA-Fa-f]+)u?\)?", hdr)}
     for name in ("SS_OK", "SS_EINVAL", "SS_ENOMEM", "SS_EIO", "SS_EDEVICE", "SS_ESTATE", "SS_EBUSY",
-                 "SS_EINTERNAL", "SS_QUERY_OK", "SS_QUERY_REJECTED", "SS_QUERY_FAULT"):
+                 "SS_EINTERNAL", "SS_QUERY_OK", "SS_QUERY_REJECTED", "SS_QUERY_FAULT", "SS_ORDER_AUTO",
+                 "SS_ORDER_INDEX", "SS_ORDER_LOCALITY", "SS_ORDER_MIN_BATCH"):
         assert name in defs, name
         assert getattr(_lib, name) == defs[name], name
     assert ctypes.sizeof(_lib.QueryStats) == 32 == STATS_DTYPE.itemsize
     assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
+    assert ctypes.sizeof(_lib.SearchParams) == 32 and _lib.SearchParams.query_order.offset == 28
+    assert "uint32_t query_order;" in hdr
