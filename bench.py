@@ -344,13 +344,13 @@ def main():
         "l_sweep": sweep,
         "graph": {"mean_degree": ginfo["passes"][-1]["mean_degree"], "knn_s": round(ginfo["knn_s"], 1),
                   "passes": len(ginfo["passes"])},
-        "roofline": roofline_record(w.name, nq, K, L, achieved_gbs, launch_bytes, kern_ms),
+        "roofline": roofline_record(w.name, nq, K, L, order_used(nq), achieved_gbs, launch_bytes, kern_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(workload, nq, K, L):
+def measured_traffic(workload, nq, K, L, order):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
     calibrated against a plain copy) for this workload and launch, from the
     record kept in profiles/search_kernel_<workload>_counters.md; None when
@@ -368,13 +368,14 @@ def measured_traffic(workload, nq, K, L):
         rec = json.loads(block[1].split("```", 1)[0])
     except ValueError:
         return None
-    if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L")) != (workload, nq, K, L):
+    if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L"), rec.get("query_order", "index")) != \
+            (workload, nq, K, L, order):
         return None
     return rec
 
 
-def roofline_record(workload, nq, K, L, achieved_gbs, launch_bytes, kern_ms):
-    rec = measured_traffic(workload, nq, K, L)
+def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_ms):
+    rec = measured_traffic(workload, nq, K, L, order)
     out = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
            "traffic": rec["traffic_bytes"] if rec else None,
