@@ -751,11 +751,14 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         if (fresh)
             w.cand_id[lanes_below(fm)] = nb;
         wave_sync();
+        // Candidate c (row order) ends in one lane, cd/cid; lanes increase
+        // with c, so lane order is row order for the merge.
         float cd = __builtin_inff();
         uint32_t cid = kNone;
+        bool has = false;
         if (nc <= 16) {
-            // Lanes 4c..4c+3 compute candidate c; lane c collects its id and
-            // distance from lane 4c.
+            // Lanes 4c..4c+3 compute candidate c and lane 4c keeps it (every
+            // lane of the team ends with the same sum).
             const uint32_t c = lane >> 2;
             uint32_t tid = kNone;
             float td[1] = {__builtin_inff()};
@@ -764,28 +767,29 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
                 const float *rows[1] = {a.base + (size_t)tid * a.dim_pad};
                 team_dists<NBLK, TAILI, METRIC, 1>(rows, w.qry, (int)(lane & 3), nblk_rt, tail_rt, td);
             }
-            const float gd = __shfl(td[0], (int)(4 * (lane & 15)), 64);
-            const uint32_t gid = (uint32_t)__shfl((int)tid, (int)(4 * (lane & 15)), 64);
-            if (lane < nc) {
-                cd = gd;
-                cid = gid;
+            if ((lane & 3u) == 0 && c < nc) {
+                cd = td[0];
+                cid = tid;
+                has = true;
             }
         } else if (nc <= 32) {
-            // Lanes 4c..4c+3 compute candidates c and c + 16 (a missing
-            // second candidate repeats the first); lane c collects from lane
-            // 4(c mod 16).
-            const uint32_t c = lane >> 2;
-            const uint32_t id0 = w.cand_id[c];
-            const uint32_t id1 = c + 16 < nc ? w.cand_id[c + 16] : id0;
+            // Lanes 4c..4c+3 compute candidates 2c and 2c + 1 (a missing one
+            // repeats candidate 0); lane 4c keeps 2c and lane 4c + 1 keeps
+            // 2c + 1.
+            const uint32_t c = lane >> 2, c0 = 2u * c, c1 = c0 + 1u;
+            const uint32_t id0 = w.cand_id[c0 < nc ? c0 : 0u];
+            const uint32_t id1 = w.cand_id[c1 < nc ? c1 : 0u];
             const float *rows[2] = {a.base + (size_t)id0 * a.dim_pad, a.base + (size_t)id1 * a.dim_pad};
             float td[2];
             team_dists<NBLK, TAILI, METRIC, 2>(rows, w.qry, (int)(lane & 3), nblk_rt, tail_rt, td);
-            const int src = (int)(4 * (lane & 15));
-            const float gd0 = __shfl(td[0], src, 64), gd1 = __shfl(td[1], src, 64);
-            const uint32_t gi0 = (uint32_t)__shfl((int)id0, src, 64), gi1 = (uint32_t)__shfl((int)id1, src, 64);
-            if (lane < nc) {
-                cd = lane < 16 ? gd0 : gd1;
-                cid = lane < 16 ? gi0 : gi1;
+            if ((lane & 3u) == 0 && c0 < nc) {
+                cd = td[0];
+                cid = id0;
+                has = true;
+            } else if ((lane & 3u) == 1 && c1 < nc) {
+                cd = td[1];
+                cid = id1;
+                has = true;
             }
         } else {
             candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
@@ -793,13 +797,14 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             if (lane < nc) {
                 cd = w.cand_d[lane];
                 cid = w.cand_id[lane];
+                has = true;
             }
         }
         st.ndc += nc;
         if (fresh)
             visit_settle(w.hash, nb, claim, st.lossy);
         wave_sync();
-        bool inA = lane < nc && cd < tail;  // strict tail filter, :238
+        bool inA = has && cd < tail;  // strict tail filter, :238
 
 #pragma unroll
         for (int c = 0; c < kMaxChunks; c++) {
