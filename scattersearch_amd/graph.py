@@ -88,10 +88,12 @@ def build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha=1.0, pas
 
 
 def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.0, passes=3, chunk=131072, nthreads=0,
-                    log=None, nn_descent_iters=0):
+                    log=None, nn_descent_iters=0, knn_chunk=0):
     """build_graph with the GPU providers of an Index over `base`.  The k-NN
     lists are exact (brute force) unless nn_descent_iters > 0, in which case
-    they come from that many NN-descent rounds."""
+    they come from that many NN-descent rounds.  knn_chunk > 0 computes the
+    exact lists that many points at a time (each point's k nearest other
+    points), with a log line per chunk."""
 
     def knn_fn(k):
         if nn_descent_iters:
@@ -99,6 +101,15 @@ def gpu_build_graph(index, base, R, knn_k, nav_L, alpha=1.0, passes=3, chunk=131
             if log:
                 log(f"nn-descent: insertions per round {upd.tolist()}")
             return ids
+        if knn_chunk:
+            n = base.shape[0]
+            out = np.empty((n, k), dtype=np.uint32)
+            for s in range(0, n, knn_chunk):
+                e = min(n, s + knn_chunk)
+                out[s:e] = index.bruteforce(base[s:e], k, exclude=np.arange(s, e, dtype=np.uint32))[0]
+                if log:
+                    log(f"knn: {e} of {n} points")
+            return out
         return index.knn_graph(k)[0]
 
     def search_fn(adj, ep, L, path_cap):
