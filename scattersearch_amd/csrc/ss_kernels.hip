@@ -805,6 +805,17 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             visit_settle(w.hash, nb, claim, st.lossy);
         wave_sync();
         bool inA = has && cd < tail;  // strict tail filter, :238
+        if constexpr (!kRegPool) {
+            // Nothing admitted: nk = L, the cursor moves on (:249-252), and
+            // the pool in LDS is not read for the merge.
+            if (ballot(inA) == 0) {
+                have = onext >= 0;
+                if (have)
+                    k = (uint32_t)onext;
+                nb = nbs;
+                continue;
+            }
+        }
 
 #pragma unroll
         for (int c = 0; c < kMaxChunks; c++) {
@@ -838,7 +849,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         }
 
         const uint64_t am = ballot(inA);
-        if (am == 0) {  // nothing admitted: nk = L, the cursor moves on (:249-252)
+        if (am == 0) {  // nothing admitted, or only pool members back (:249-252)
             have = onext >= 0;
             if (have)
                 k = (uint32_t)onext;
