@@ -1,6 +1,7 @@
-"""GPU parity at a larger scale, on a graph made by the GPU graph builder
+"""GPU parity at a larger scale, on graphs made by the GPU graph builder
 (exact k-NN, search-path candidates, MRNG prune, reverse edges): 200,000 x
-128 points from a 200-component mixture, 2,000 queries.  Ids, distances
+128 points from a 200-component mixture, 2,000 queries (L2); 100,000 x 200
+unit-norm points, 1,000 queries with mixed K and L (inner product).  Ids, distances
 (bit for bit) and hop counts must equal the CPU oracle's
 (SearchArraySimulation, src/annsearch.cpp:182-259); distance counts too
 unless the visited hash forgot an id (a lossy insert adds recomputations
@@ -44,3 +45,37 @@ def test_mid_scale_recall(mid):
     gt, _ = mid["ix"].bruteforce(mid["queries"], 10)
     ids, _, _ = mid["ix"].search(mid["queries"], 10, 128)
     assert metrics.recall(ids, gt, 10) >= 0.9
+
+
+@pytest.fixture(scope="module")
+def mid_ip():
+    """100,000 x 200 unit-norm points (log-normal norms before normalising, 200
+    components), inner product, R=64 graph from the GPU builder."""
+    from scattersearch_amd import Index, data
+    from scattersearch_amd.graph import gpu_build_graph
+    base, queries = data.mixture(100_000, 1_000, 200, 200, 0.25, seed=20261231, lognormal_sigma=0.5,
+                                 unit_norm=True)
+    ix = Index(base, 1)
+    adj, ep, _ = gpu_build_graph(ix, base, 64, 64, 64)
+    return dict(ix=ix, base=base, queries=queries, adj=adj, ep=ep)
+
+
+def test_mid_scale_inner_product_mixed_k_and_l(mid_ip):
+    """One launch mixing K in {1, 10, 100} with L uniform in K..512 (pools up to
+    eight 64-entry chunks in LDS), inner product: ids, distance bits and hops
+    equal the oracle's for every query."""
+    import oracle
+    rng = np.random.default_rng(3)
+    nq = mid_ip["queries"].shape[0]
+    K = rng.choice(np.array([1, 10, 100], dtype=np.uint32), size=nq)
+    L = np.array([rng.integers(int(k), 513) for k in K], dtype=np.uint32)
+    ids, ds, st = mid_ip["ix"].search(mid_ip["queries"], K, L)
+    o_ids, o_d, o_ndc, o_hops = oracle.search_batch(mid_ip["base"], mid_ip["adj"], mid_ip["ep"],
+                                                    mid_ip["queries"], K, L, metric=1)
+    bad = np.nonzero((ids != o_ids).any(1))[0]
+    assert bad.size == 0, f"ids differ on queries {bad[:10]}"
+    assert np.array_equal(ds.view(np.uint32), o_d.view(np.uint32)), "distances differ"
+    assert np.array_equal(st["hops"], o_hops), "hop counts differ"
+    exact = st["lossy"] == 0
+    assert np.array_equal(st["ndc"][exact], o_ndc[exact]), "distance counts differ"
+    assert (st["status"] == 0).all()
