@@ -457,7 +457,7 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
     def launch():
         ix.search_device(dq.data_ptr(), nq, k_max, l_max, k_max, d_ids.data_ptr(), d_dist.data_ptr(),
                          d_stats.data_ptr(), d_K=dK.data_ptr(), d_L=dL.data_ptr(), stream=stream.cuda_stream,
-                         use_graph=True)
+                         use_graph=True, visited_slots=args.visited_slots, waves_per_block=args.waves_per_block)
 
     for _ in range(args.warmup):
         launch()

@@ -425,6 +425,8 @@ int ss_load_graph(ss_index *ix, const char *path)
 struct Shape {
     uint32_t wpb, slots, blocks;
     size_t lds_wave;
+    bool hash16;  // 16-bit visited slots (one wavefront per query)
+    uint32_t hash_rem_bits, hash_id_mask;
 };
 
 
@@ -488,9 +490,14 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         s->slots = slots;
         s->blocks = blocks;
         s->lds_wave = per_block / wq;
+        s->hash16 = false;
+        s->hash_rem_bits = s->hash_id_mask = 0;
         return SS_OK;
     }
-    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, slots, ew);
+    s->hash_rem_bits = s->hash_id_mask = 0;
+    s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu, &s->hash_rem_bits,
+                               &s->hash_id_mask);
+    const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots, ew);
     const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
     if (waves_cu == 0) {
         set_error("search state does not fit in LDS: lower L or visited_slots");
@@ -515,6 +522,37 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
 }  // extern "C"
 
 namespace ss {
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *id_mask)
+{
+    uint32_t sb = 0;  // log2(slots)
+    while ((1u << sb) < slots)
+        sb++;
+    if (sb < 3)
+        return false;
+    const uint32_t bb = sb - 3;  // buckets of eight slots
+    uint32_t n = 0;  // ceil(log2(num))
+    while (n < 32 && (1ull << n) < num)
+        n++;
+    if (n < bb)
+        n = bb;
+    if (n > 31 || n - bb > 15)
+        return false;
+    *rem_bits = n - bb;
+    *id_mask = (1u << n) - 1u;
+    return true;
+}
+
+bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
+                size_t lds_per_cu, uint32_t *rem_bits, uint32_t *id_mask)
+{
+    if (l_max <= 64 || expand_width != 1)  // the kernels for these keep the 32-bit table
+        return false;
+    const size_t lds_cu = lds_per_cu ? lds_per_cu : 65536;
+    if (lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots, expand_width) >= 16)
+        return false;
+    return hash_params16(num, slots, rem_bits, id_mask);
+}
+
 int index_view(const ss_index *ix, IndexView *v)
 {
     if (!ix || !v) {
@@ -643,6 +681,9 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.l_max = l_max;
     a.hash_slots = sh.slots;
     a.hash_shift = 32u - log2u(sh.slots);
+    a.hash16 = sh.hash16 ? 1u : 0u;
+    a.hash_rem_bits = sh.hash_rem_bits;
+    a.hash_id_mask = sh.hash_id_mask;
     a.k_stride = k_stride;
     a.q_first = 0;
     a.out_ids = d_out_ids;

@@ -29,6 +29,9 @@ struct SearchArgs {
     uint32_t l_max;       // sizes the LDS pool (<= SS_MAX_L)
     uint32_t hash_shift;  // 32 - log2(hash slots)
     uint32_t hash_slots;
+    uint32_t hash16;        // 1: 16-bit visited slots (hash_params16), one wavefront per query only
+    uint32_t hash_rem_bits; // 16-bit slots: bits of the hashed id kept in a slot (<= 15)
+    uint32_t hash_id_mask;  // 16-bit slots: ids are hashed modulo this + 1 (a power of two)
     uint32_t k_stride;
     uint32_t q_first;     // first query index this launch handles
     uint32_t *out_ids;
@@ -120,7 +123,9 @@ struct BruteArgs {
 };
 
 // Per-wave LDS bytes for a search with the given pool bound and hash size.
-size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+// hash_words: 32-bit words of the visited table (slots, or slots / 2 for
+// 16-bit slots).
+size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_words,
                                  uint32_t expand_width = 1);
 // LDS bytes of one block (one query) of the multi-wavefront search.
 size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
@@ -170,6 +175,20 @@ struct IndexView {
     bool has_graph;
 };
 int index_view(const ss_index *ix, IndexView *v);
+// Whether a visited hash of `slots` slots (buckets of eight) can hold the ids
+// of an index of `num` points in 16-bit slots: with b = log2(slots / 8) and
+// n = max(ceil(log2(num)), b), id -> (id * 0x9E3779B1) mod 2^n is a bijection
+// whose top b bits pick the bucket and whose low n - b bits are stored, which
+// needs n - b <= 15 (bit 15 marks an empty slot).
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *id_mask);
+// Whether a one-wavefront-per-query launch uses 16-bit slots: for pools above
+// 64 entries expanded one node per step, when they fit (hash_params16) and
+// the 32-bit table leaves room for fewer than 16 wavefronts per CU (the most
+// the search kernels keep resident), so that the smaller table raises
+// residency; otherwise the 32-bit table, whose lookup and claim take fewer
+// instructions and LDS round trips.
+bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
+                size_t lds_per_cu, uint32_t *rem_bits, uint32_t *id_mask);
 // Launch shape (waves per block, visited slots, blocks) the batched search
 // would use for nq queries with pool bound l_max; 0 arguments pick defaults.
 int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,

@@ -363,6 +363,83 @@ __device__ __forceinline__ bool visit(uint32_t *tab, uint32_t bshift, uint32_t i
     return fresh;
 }
 
+// The same set in 16-bit slots, for a table only one wavefront writes (one
+// wavefront per query): a bucket of eight slots is one 16-byte LDS read.
+// h = (id * 0x9E3779B1) & id_mask is a bijection of the ids; the bucket is
+// h >> rem_bits and the slot keeps h's low rem_bits (<= 15) bits, so a slot
+// value and its bucket give the id back exactly.  An empty slot is 0xFFFF,
+// the only value with bit 15 set.  A claim is a plain 16-bit store; lanes of
+// one step that stored into the same slot are sorted out by reading the slot
+// back in visit16_settle (the last store wins, the others move on).
+__device__ __forceinline__ bool visit16_begin(uint32_t *tab, uint32_t rem_bits, uint32_t id_mask, uint32_t id,
+                                              VisitClaim &c, uint32_t &lossy)
+{
+    c.slot = kNone;
+    c.old = kNone;
+    const uint32_t h = (id * 0x9E3779B1u) & id_mask;
+    const uint32_t b = h >> rem_bits;
+    const uint32_t rem = h & ((1u << rem_bits) - 1u);
+    const uint4 x = reinterpret_cast<const uint4 *>(tab)[b];
+    // Present: a 16-bit half of some word XOR (rem, rem) is zero (the borrow
+    // of v - 0x00010001 reaches bit 15 of a half only through a zero half).
+    const uint32_t rr = rem | (rem << 16);
+    const uint32_t y0 = x.x ^ rr, y1 = x.y ^ rr, y2 = x.z ^ rr, y3 = x.w ^ rr;
+    const uint32_t z = ((y0 - 0x00010001u) & ~y0) | ((y1 - 0x00010001u) & ~y1) | ((y2 - 0x00010001u) & ~y2) |
+                       ((y3 - 0x00010001u) & ~y3);
+    if (z & 0x80008000u)
+        return false;
+    // Slot s is half s & 1 of word s >> 1; free slots have bit 15 set.
+    const uint32_t fm = ((x.x >> 15) & 1u) | ((x.x >> 30) & 2u) | ((x.y >> 13) & 4u) | ((x.y >> 28) & 8u) |
+                        ((x.z >> 11) & 16u) | ((x.z >> 26) & 32u) | ((x.w >> 9) & 64u) | ((x.w >> 24) & 128u);
+    if (fm == 0) {
+        lossy++;
+        return true;
+    }
+    const uint32_t s0 = id & (kBucket - 1);
+    const uint32_t f2 = fm | (fm << kBucket);
+    c.slot = b * kBucket + ((s0 + (uint32_t)__builtin_ctz(f2 >> s0)) & (kBucket - 1));
+    c.old = rem;
+    reinterpret_cast<uint16_t *>(tab)[c.slot] = (uint16_t)rem;
+    return true;
+}
+
+__device__ __forceinline__ void visit16_settle(uint32_t *tab, const VisitClaim &c, uint32_t &lossy)
+{
+    if (c.slot == kNone)
+        return;
+    volatile uint16_t *t16 = reinterpret_cast<volatile uint16_t *>(tab);
+    const uint32_t rem = c.old;
+    if (t16[c.slot] == rem)
+        return;
+    const uint32_t b8 = c.slot & ~(kBucket - 1);
+    for (uint32_t i = 1; i < kBucket; i++) {
+        const uint32_t s = b8 + ((c.slot + i) & (kBucket - 1));
+        if (t16[s] != 0xFFFFu)
+            continue;
+        t16[s] = (uint16_t)rem;
+        if (t16[s] == rem)
+            return;
+    }
+    lossy++;
+}
+
+__device__ __forceinline__ bool visit_any(const SearchArgs &a, uint32_t *tab, uint32_t id, uint32_t &lossy)
+{
+    if (a.hash16) {
+        VisitClaim c;
+        const bool fresh = visit16_begin(tab, a.hash_rem_bits, a.hash_id_mask, id, c, lossy);
+        visit16_settle(tab, c, lossy);
+        return fresh;
+    }
+    return visit(tab, a.hash_shift + 3u, id, lossy);
+}
+
+// 32-bit words of one wavefront's visited table.
+__host__ __device__ inline uint32_t hash_words(const SearchArgs &a)
+{
+    return a.hash16 ? a.hash_slots / 2u : a.hash_slots;
+}
+
 struct WaveLds {
     float *qry;
     uint32_t *pool_id;
@@ -374,10 +451,12 @@ struct WaveLds {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t wave_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
+// hash_words: 32-bit words of the visited table (its slots, or half of them
+// for 16-bit slots); the table is the last region of a wavefront's share.
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_words,
                                                  uint32_t ew = 1)
 {
-    return align16(dim_pad * 4u) + align16(l_max * 4u) * 2 + 64 * 4 * 2 * ew + align16(hash_slots * 4u);
+    return align16(dim_pad * 4u) + align16(l_max * 4u) * 2 + 64 * 4 * 2 * ew + align16(hash_words * 4u);
 }
 
 __device__ inline WaveLds carve(unsigned char *p, uint32_t dim_pad, uint32_t l_max, uint32_t ew = 1)
@@ -512,7 +591,7 @@ __device__ __forceinline__ bool query_prologue(const SearchArgs &a, const WaveLd
             qdst[i] = qsrc[i];
     }
     uint4 *h4 = reinterpret_cast<uint4 *>(w.hash);
-    for (uint32_t i = lane; i < a.hash_slots / 4; i += 64)
+    for (uint32_t i = lane; i < hash_words(a) / 4; i += 64)
         h4[i] = make_uint4(kNone, kNone, kNone, kNone);
     return true;
 }
@@ -531,7 +610,7 @@ __device__ __forceinline__ void seed_pool(const SearchArgs &a, const WaveLds &w,
     const uint32_t deg = (uint32_t)__popcll(ballot(nb != kNone));
     const uint32_t t = deg < st.L ? deg : st.L;
     if (lane < t) {
-        visit(w.hash, a.hash_shift + 3u, nb, st.lossy);
+        visit_any(a, w.hash, nb, st.lossy);
         w.cand_id[lane] = nb;
     }
     wave_sync();
@@ -627,6 +706,9 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     const int nblk_rt = (int)(a.dim_pad / 16);
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
     const uint32_t bshift = a.hash_shift + 3u;  // buckets of eight slots
+    // 16-bit slots are only ever chosen for pools above 64 entries, so the
+    // register-pool body keeps the 32-bit table alone.
+    const bool h16 = MC > 1 && a.hash16 != 0;
     const uint32_t L = st.L;
 
     // k is the pool position of the node to expand and nb already holds its
@@ -745,7 +827,9 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
 
         // Visited test of the whole row, compaction of the fresh ids.
         VisitClaim claim{kNone, kNone};
-        const bool fresh = nb != kNone && visit_begin(w.hash, bshift, nb, claim, st.lossy);
+        const bool fresh =
+            nb != kNone && (h16 ? visit16_begin(w.hash, a.hash_rem_bits, a.hash_id_mask, nb, claim, st.lossy)
+                                : visit_begin(w.hash, bshift, nb, claim, st.lossy));
         const uint64_t fm = ballot(fresh);
         const uint32_t nc = (uint32_t)__popcll(fm);
         if (fresh)
@@ -801,8 +885,12 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
             }
         }
         st.ndc += nc;
-        if (fresh)
-            visit_settle(w.hash, nb, claim, st.lossy);
+        if (fresh) {
+            if (h16)
+                visit16_settle(w.hash, claim, st.lossy);
+            else
+                visit_settle(w.hash, nb, claim, st.lossy);
+        }
         wave_sync();
         bool inA = has && cd < tail;  // strict tail filter, :238
         if constexpr (!kRegPool) {
@@ -1018,6 +1106,7 @@ __device__ void search_wide(const SearchArgs &a, const WaveLds &w, uint32_t q)
         uint32_t nc = 0;
 #pragma unroll
         for (int r = 0; r < EW; r++) {
+            // (expand_width > 1 always runs with the 32-bit table)
             const bool fresh = nb[r] != kNone && visit(w.hash, a.hash_shift + 3u, nb[r], st.lossy);
             const uint64_t fm = ballot(fresh);
             if (fresh)
@@ -1151,7 +1240,7 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t waves = blockDim.x >> 6;
-    const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, EW);
+    const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, hash_words(a), EW);
     const WaveLds w = carve(smem + per_wave * wave, a.dim_pad, a.l_max, EW);
     // One call site for the search body (two would keep it out of line).
     bool first = true;
@@ -1869,7 +1958,7 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(Se
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wave = threadIdx.x >> 6;
     const SearchArgs &sa = a.search;
-    const size_t per_wave = wave_lds_bytes(sa.dim_pad, sa.l_max, sa.hash_slots, 1);
+    const size_t per_wave = wave_lds_bytes(sa.dim_pad, sa.l_max, hash_words(sa), 1);
     const WaveLds w = carve(smem + per_wave * wave, sa.dim_pad, sa.l_max, 1);
     const uint32_t lane = lane_id();
     bool standby = false;  // counted in ctl.standby
@@ -2382,7 +2471,7 @@ struct SearchLaunch {
     template <int EW, int MC>
     static hipError_t go(const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
-        const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, a.hash_slots, EW) * wpb;
+        const size_t lds = wave_lds_bytes(a.dim_pad, a.l_max, hash_words(a), EW) * wpb;
         hipLaunchKernelGGL((search_kernel<NBLK, TAILI, METRIC, EW, MC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
         return hipGetLastError();
     }
@@ -2452,7 +2541,7 @@ struct ServeLaunch {
     template <int MC>
     static hipError_t go(const ServeArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t st)
     {
-        const size_t lds = wave_lds_bytes(a.search.dim_pad, a.search.l_max, a.search.hash_slots, 1) * wpb;
+        const size_t lds = wave_lds_bytes(a.search.dim_pad, a.search.l_max, hash_words(a.search), 1) * wpb;
         hipLaunchKernelGGL((serve_kernel<NBLK, TAILI, METRIC, MC>), dim3(blocks), dim3(64 * wpb), lds, st, a);
         return hipGetLastError();
     }
@@ -2514,9 +2603,9 @@ hipError_t attr_all(size_t max_lds)
 
 }  // namespace
 
-size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots, uint32_t expand_width)
+size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash_words, uint32_t expand_width)
 {
-    return wave_lds_bytes(dim_pad, l_max, hash_slots, expand_width);
+    return wave_lds_bytes(dim_pad, l_max, hash_words, expand_width);
 }
 
 size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots, uint32_t waves_per_query)
