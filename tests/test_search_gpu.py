@@ -269,3 +269,22 @@ def test_autotune_exact_and_relaxed(small):
     assert relaxed["waves_per_query"] in (1, 2, 4)
     r_tuned = metrics.recall(ix.search(small["queries"], 10, 48)[0], gt, 10)  # relaxed parameters apply
     assert r_tuned >= r_exact - 0.05 - 1e-6
+
+
+def test_small_16bit_table_with_lossy_inserts():
+    """2048-d vectors and L=512 leave room for fewer than 16 wavefronts per CU,
+    so the launch keeps 16-bit visited slots; 256 slots make lanes of one step
+    collide on slots and fill buckets (lossy inserts).  Ids, distance bits and
+    hops still equal the oracle's; distance counts wherever nothing was lost."""
+    import oracle
+    from scattersearch_amd import data
+    base, q = data.mixture(3000, 40, 2048, 8, 0.3, seed=2048)
+    adj, ep, _ = oracle.cpu_graph(base, 0, 24, 24, 32, passes=1)
+    ix = _index(base, adj, ep)
+    gpu = ix.search(q, 10, 512, visited_slots=256)
+    assert gpu[2]["lossy"].sum() > 0
+    ora = oracle.search_batch(base, adj, ep, q, 10, 512)
+    _assert_same(gpu, ora, check_ndc=False, what="16-bit table, 256 slots")
+    exact = gpu[2]["lossy"] == 0
+    assert np.array_equal(gpu[2]["ndc"][exact], ora[2][exact])
+    assert (gpu[2]["ndc"] >= ora[2]).all()
