@@ -1713,9 +1713,17 @@ __device__ void steal_entries(const SearchArgs &a, const ScatterLds &s, uint32_t
         candidate_distances<NBLK, TAILI, METRIC>(a, w, nc, nblk_rt, tail_rt);
         wave_sync();
         st.ndc += nc;
-        // sort the batch by (distance, claim order)
-        const float cd = lane < nc ? w.cand_d[lane] : __builtin_inff();
+        // A victim moving its entries (its merge rewrites them with plain
+        // stores) can show one id at two positions: keep the first claim.
+        // Then sort the batch by (distance, claim order), repeats last.
         const uint32_t cid = lane < nc ? w.cand_id[lane] : kNone;
+        bool dup = false;
+        for (uint32_t j = 0; j < nc; j++) {
+            const uint32_t x = readlane_u(cid, j);
+            dup = dup || (j < lane && x == cid);
+        }
+        const float cd = lane < nc && !dup ? w.cand_d[lane] : __builtin_inff();
+        const uint32_t nkeep = nc - (uint32_t)__popcll(ballot(lane < nc && dup));
         uint32_t rank = 0;
         for (uint32_t j = 0; j < nc; j++) {
             const float dj = readlane_f(cd, j);
@@ -1727,7 +1735,7 @@ __device__ void steal_entries(const SearchArgs &a, const ScatterLds &s, uint32_t
             w.cand_d[rank] = cd;
         }
         wave_sync();
-        const uint32_t t = merge_sorted(w.pool_id, w.pool_d, st.t, w.cand_id, w.cand_d, nc, st.t, s.tmp_id, s.tmp_d);
+        const uint32_t t = merge_sorted(w.pool_id, w.pool_d, st.t, w.cand_id, w.cand_d, nkeep, st.t, s.tmp_id, s.tmp_d);
         copy_list(s.tmp_id, s.tmp_d, t, w.pool_id, w.pool_d);
     }
 }
