@@ -58,8 +58,10 @@ def lib():
                                      _f32p, ctypes.c_int]
         L.ora_scatter_search.restype = ctypes.c_int
         L.ora_scatter_search.argtypes = [ctypes.POINTER(Graph), _f32p, _u32, ctypes.c_int, _f32p, _u32,
-                                         _u32, _u32, ctypes.c_int, _u32, _u32p, _f32p,
+                                         _u32, _u32, ctypes.c_int, _u32, _u32p, _f32p, _f32p,
                                          ctypes.POINTER(ctypes.c_uint64)]
+        L.ora_scatter_entry.restype = _u32
+        L.ora_scatter_entry.argtypes = [_u32, _u32, ctypes.c_int, _u32]
         _lib = L
     return _lib
 
@@ -148,21 +150,29 @@ def bruteforce(base, queries, k, metric=L2, exclude=None, nthreads=0):
     return ids, ds
 
 
-def scatter_search(base, adj, ep, queries, K, L, nthreads, metric=L2, seed=1):
+def scatter_search(base, adj, ep, queries, K, L, nthreads, metric=L2, seed=1, dists=False):
     """ScatterSearch intra-query search, queries one after another; returns
-    (ids [nq][K], latency_ms [nq], total_ndc)."""
+    (ids [nq][K], latency_ms [nq], total_ndc), and the distances [nq][K]
+    last when dists is true."""
     base = np.ascontiguousarray(base, dtype=np.float32)
     Q = np.ascontiguousarray(queries, dtype=np.float32)
     nq = Q.shape[0]
     g, keep = _graph(adj, ep)
     ids = np.empty((nq, K), dtype=np.uint32)
+    ds = np.empty((nq, K), dtype=np.float32) if dists else None
     lat = np.zeros(nq, dtype=np.float32)
     ndc = ctypes.c_uint64()
     rc = lib().ora_scatter_search(ctypes.byref(g), _f(base), base.shape[1], metric, _f(Q), nq, K, L,
-                                  nthreads, seed, _u(ids), _f(lat), ctypes.byref(ndc))
+                                  nthreads, seed, _u(ids), _f(ds) if dists else None, _f(lat),
+                                  ctypes.byref(ndc))
     if rc:
         raise RuntimeError(f"ora_scatter_search failed ({rc})")
-    return ids, lat, ndc.value
+    return (ids, lat, ndc.value, ds) if dists else (ids, lat, ndc.value)
+
+
+def scatter_entry(seed, qidx, thread, num):
+    """The entry point thread `thread` of scatter_search draws for query qidx."""
+    return lib().ora_scatter_entry(seed, qidx, thread, num)
 
 
 def insert_into_pool(pool, K, nn):

@@ -562,7 +562,28 @@ static void et_until_best_stops(et_shared *S, shared_nb *retset, uint32_t *tmp_l
     et_expand_loop(S, retset, tmp_l, &hop, ndc, tid, &dummy, 0);      /* :1418-1454 */
 }
 
-static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *out, uint64_t *ndc_total)
+/* The per-thread random state of one query (the reference calls rand(),
+ * src/annsearch.cpp:876); its first draw is the thread's entry point. */
+/* The state is the splitmix64 finaliser of (seed, query, thread), so that
+ * the threads' first draws are independent (a plain xor of the thread number
+ * into a xorshift state reaches only the low bits of its first output). */
+static uint64_t et_rng_init(uint32_t seed, uint32_t qidx, int thread)
+{
+    uint64_t z = ((uint64_t)seed << 32 | qidx) * 0x9E3779B97F4A7C15ull + (uint64_t)(thread + 1) * 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z ? z : 1;
+}
+
+uint32_t ora_scatter_entry(uint32_t seed, uint32_t qidx, int thread, uint32_t num)
+{
+    uint64_t rng = et_rng_init(seed, qidx, thread);
+    return num ? xs_next(&rng) % num : ORA_NONE;
+}
+
+static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *out, float *out_d,
+                        uint64_t *ndc_total)
 {
     const int T = S->T;
     const uint32_t L = S->L, K = S->K;
@@ -576,14 +597,15 @@ static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *ou
     memset(S->good_thread, 0, sizeof(int) * T);
     memset(S->is_reach, 0, T);
     uint32_t final_tmp_l[256];
+    int team = T;
 #pragma omp parallel num_threads(T)
     {
         int i = omp_get_thread_num();
+        if (i == 0 && omp_get_num_threads() != T)
+            team = omp_get_num_threads();  /* fewer threads than asked: reported below */
         shared_nb *retset = S->retsets[i];
         memset(retset, 0, sizeof(shared_nb) * (L + 1));
-        uint64_t rng = 0x9E3779B97F4A7C15ull ^ ((uint64_t)seed << 32) ^ ((uint64_t)qidx * 0x100000001B3ull) ^ (uint64_t)(i + 1);
-        if (!rng)
-            rng = 1;
+        uint64_t rng = et_rng_init(seed, qidx, i);
         int hop = 0, need_identify = 1;
         uint64_t ndc = 0;
         uint32_t tmp_l = 0;
@@ -624,6 +646,8 @@ static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *ou
         final_tmp_l[i] = tmp_l;
         ndc_thread[i] = ndc;
     }
+    if (team != T)
+        return -3;
     int best = S->best_thread_id;
     if (best < 0) {
         float bd = INFINITY;
@@ -645,8 +669,12 @@ static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *ou
                 break;
         }
     }
-    for (uint32_t j = 0; j < K; j++)                                   /* :1004-1007 */
-        out[j] = j < final_tmp_l[best] ? LD(S->retsets[best][j].id) : ORA_NONE;
+    for (uint32_t j = 0; j < K; j++) {                                 /* :1004-1007 */
+        const int has = j < final_tmp_l[best];
+        out[j] = has ? LD(S->retsets[best][j].id) : ORA_NONE;
+        if (out_d)
+            out_d[j] = has ? nb_dist(&S->retsets[best][j]) : INFINITY;
+    }
     memset(S->flags, 0, S->g->num);                                   /* :1008 */
     for (int i = 0; i < T; i++)
         *ndc_total += ndc_thread[i];
@@ -656,7 +684,7 @@ static int et_one_query(et_shared *S, uint32_t qidx, uint32_t seed, uint32_t *ou
 int ora_scatter_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
                        const float *queries, uint32_t nq, uint32_t K, uint32_t L,
                        int nthreads, uint32_t seed,
-                       uint32_t *out_ids, float *latency_ms, uint64_t *total_ndc)
+                       uint32_t *out_ids, float *out_dists, float *latency_ms, uint64_t *total_ndc)
 {
     if (nthreads <= 0)
         nthreads = omp_get_max_threads();
@@ -687,7 +715,8 @@ int ora_scatter_search(const ora_graph *g, const float *base, uint32_t dim, int 
     for (uint32_t q = 0; !rc && q < nq; q++) {                         /* tests/parallel_search.cpp:75-87 */
         S.query = queries + (size_t)q * dim;
         double t0 = now_ms();
-        rc = et_one_query(&S, q, seed, out_ids + (size_t)q * K, &ndc);
+        rc = et_one_query(&S, q, seed, out_ids + (size_t)q * K, out_dists ? out_dists + (size_t)q * K : NULL,
+                          &ndc);
         if (latency_ms)
             latency_ms[q] = (float)(now_ms() - t0);
     }

@@ -19,8 +19,18 @@
  *                               here (it needs Boost), so its traversal is
  *                               pinned only through those checks.
  *   ora_bruteforce              closed form: sorted by (dist, id), exact.
- *   ora_scatter_search          CPU baseline timing only; non-deterministic
- *                               like the reference (parity unpinned).
+ *   ora_scatter_search          non-deterministic like the reference, pinned
+ *                               where the mathematics fixes the outcome
+ *                               (tests): with one thread it is
+ *                               ora_search from the entry point the thread
+ *                               draws (ora_scatter_entry), ids, distances
+ *                               and distance count; on a complete graph
+ *                               with L >= N-1 and two or more distinct
+ *                               entry points, every thread count returns the
+ *                               exact top-K with T*(N-1) distances; at
+ *                               every thread count the ids are real points,
+ *                               unique, in order of exact distance.  Races
+ *                               checked with ThreadSanitizer (Archer).
  */
 #ifndef SS_ORACLE_H
 #define SS_ORACLE_H
@@ -85,11 +95,18 @@ int ora_bruteforce(const float *base, uint32_t num, const float *queries, uint32
 
 /* ScatterSearch intra-query search with election-based early termination
  * (src/annsearch.cpp:839-1025, 1368-1455), queries run one after another,
- * each with nthreads threads sharing one visited array.  Timing baseline. */
+ * each with nthreads threads sharing one visited array.  Timing baseline.
+ * out_ids / out_dists (optional) are [nq][K], (ORA_NONE, +inf) past the
+ * filled pool.  Returns -3 if OpenMP runs the query with fewer threads than asked. */
 int ora_scatter_search(const ora_graph *g, const float *base, uint32_t dim, int metric,
                        const float *queries, uint32_t nq, uint32_t K, uint32_t L,
                        int nthreads, uint32_t seed,
-                       uint32_t *out_ids, float *latency_ms, uint64_t *total_ndc);
+                       uint32_t *out_ids, float *out_dists, float *latency_ms, uint64_t *total_ndc);
+
+/* The entry point thread `thread` of ora_scatter_search draws for query
+ * number qidx of a call with this seed (the reference draws rand() % num,
+ * src/annsearch.cpp:876). */
+uint32_t ora_scatter_entry(uint32_t seed, uint32_t qidx, int thread, uint32_t num);
 
 #ifdef __cplusplus
 }

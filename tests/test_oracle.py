@@ -225,6 +225,69 @@ def test_bruteforce_closed_form():
         assert np.array_equal(ds[r], d[order])
 
 
+def test_scatter_one_thread_is_the_plain_search(c0):
+    """With one thread ScatterSearch (src/annsearch.cpp:839-1025) elects
+    itself at hop 50, never steals and merges nothing, so it is the plain
+    search (src/annsearch.cpp:182-259) from the entry point it draws:
+    identical ids, distance bits and distance count.  Scaling the data by 30
+    puts every K-th distance above the reference's initial best_dist of 1000
+    (:850), so no thread is elected and the search goes on in the stealing
+    phase (:1368-1455), which must give the same result again."""
+    base0, adj = c0["base"], c0["adj"]
+    q0 = c0["queries"][:60]
+    seed = 5
+    for scale in (1.0, 30.0):
+        base = (base0 * np.float32(scale)).astype(np.float32)
+        q = (q0 * np.float32(scale)).astype(np.float32)
+        for K, L in ((10, 64), (1, 40), (32, 96)):
+            ids, lat, ndc, ds = oracle.scatter_search(base, adj, c0["ep"], q, K, L, 1, seed=seed, dists=True)
+            want_ndc = 0
+            for i in range(q.shape[0]):
+                ep = oracle.scatter_entry(seed, i, 0, base.shape[0])
+                o_ids, o_d, o_ndc, _ = oracle.search(base, adj, ep, q[i], K, L)
+                assert np.array_equal(ids[i], o_ids), (scale, K, L, i)
+                assert np.array_equal(ds[i].view(np.uint32), o_d.view(np.uint32)), (scale, K, L, i)
+                want_ndc += o_ndc
+            assert ndc == want_ndc, (scale, K, L)
+
+
+@pytest.mark.parametrize("T", [2, 4, 8])
+def test_scatter_complete_graph_is_exact(T):
+    """On a complete graph with L >= N-1 every thread seeds with every point
+    but its entry (N-1 distances per thread); an expansion can only find a
+    thread's own entry point not yet marked by the others' seeds (at most
+    one more distance per thread, depending on timing).  The winner's pool
+    merged with the others' top K (src/annsearch.cpp:990-999) is then the
+    exact top K whenever two threads drew different entry points, whatever
+    the timing."""
+    rng = np.random.default_rng(31 + T)
+    num, dim, nq, K, L, seed = 200, 12, 12, 10, 256, 3
+    base = rng.standard_normal((num, dim)).astype(np.float32)
+    q = rng.standard_normal((nq, dim)).astype(np.float32)
+    adj = np.full((num, num - 1), NONE, dtype=np.uint32)
+    for n in range(num):
+        adj[n] = rng.permutation([m for m in range(num) if m != n])
+    for i in range(nq):
+        assert len({oracle.scatter_entry(seed, i, t, num) for t in range(T)}) >= 2
+    ids, lat, ndc, ds = oracle.scatter_search(base, adj, 0, q, K, L, T, seed=seed, dists=True)
+    bf, bd = oracle.bruteforce(base, q, K)
+    assert np.array_equal(ids, bf)
+    assert np.array_equal(ds.view(np.uint32), bd.view(np.uint32))
+    assert nq * T * (num - 1) <= ndc <= nq * T * num
+
+
+@pytest.mark.parametrize("T", [2, 4, 8])
+def test_scatter_results_are_real_points_in_order(c0, T):
+    base, q = c0["base"], c0["queries"][:50]
+    ids, lat, ndc, ds = oracle.scatter_search(base, c0["adj"], c0["ep"], q, 10, 64, T, dists=True)
+    assert (ids < base.shape[0]).all()
+    for i in range(q.shape[0]):
+        assert len(set(ids[i].tolist())) == 10, i
+        assert (np.diff(ds[i]) >= 0).all(), i
+        for j in range(10):
+            assert ds[i, j] == np.float32(oracle.dist(0, q[i], base[ids[i, j]])), (i, j)
+
+
 def test_scatter_search_runs_and_finds_neighbours(c0):
     gt, _ = oracle.bruteforce(c0["base"], c0["queries"][:40], 10)
     ids, lat, ndc = oracle.scatter_search(c0["base"], c0["adj"], c0["ep"], c0["queries"][:40], 10, 64, 4)
