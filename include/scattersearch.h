@@ -63,7 +63,7 @@ typedef struct ss_search_params {
                                  together (ModifiedDeltaStepping,
                                  src/annsearch.cpp:1586-1731): fewer dependent
                                  steps, recall instead of exact parity */
-    uint32_t waves_per_query; /* 1 (default): one wavefront per query; 2..4:
+    uint32_t waves_per_query; /* 1 (default): one wavefront per query; 2..8:
                                  ScatterSearch (MultiThreadSearchArraySimulation-
                                  WithET, src/annsearch.cpp:839-1025): that many
                                  wavefronts per query sharing the visited set,
@@ -106,7 +106,11 @@ const char *ss_last_error(void);
 
 /* ANNSearch::ANNSearch(dim, num, base, Metric) (src/annsearch.cpp:11-58).
  * Unlike the reference, which borrows `base`, the index copies it into HBM
- * and owns that copy; `base` is row-major [num][dim] on the host. */
+ * and owns that copy; `base` is row-major [num][dim] on the host.  A NaN
+ * or infinite coordinate is refused (SS_EINVAL), here and in every query
+ * passed through host memory (ss_search, ss_search_path, ss_bruteforce,
+ * ss_submit); vectors already on the device are not scanned: non-finite
+ * ones there give unspecified results (never an access outside the index). */
 int ss_create(uint32_t dim, uint32_t num, const float *base, int metric, ss_index **out);
 
 /* Same, with the base already on the device as [num][dim_pad] fp32
@@ -128,7 +132,9 @@ int ss_index_info(const ss_index *index, uint32_t *dim, uint32_t *num, uint32_t 
  * Unlike the reference, a missing or malformed file, an id >= num, a node
  * count different from num or a degree above SS_MAX_RPAD is an error.
  * Repeated ids inside one row are dropped (the reference's visited set
- * skips them anyway). */
+ * skips them wherever a row is expanded), except in the entry point's row,
+ * whose first min(L, deg) entries seed the pool as they are, repeats
+ * included, as in the reference (src/annsearch.cpp:187-214). */
 int ss_load_graph(ss_index *index, const char *nsg_path);
 
 /* Install a graph from host memory: adj is [num][r_pad] u32, rows

@@ -5,6 +5,14 @@
 // GPU launch and prints L,Throughput,latency,dist_comps,recall,p95recall,
 // p99recall (tests/search.cpp:67,113) plus p50/p99 per-query service time.
 // The metric is an argument here; the reference hard-codes INNER_PRODUCT.
+//
+// For each L the first (up to 200) queries are also searched one call at a
+// time through ANNSearch::SearchArraySimulation, the reference's per-query
+// call (tests/search.cpp:69-85), and must return the batch's ids: a
+// difference is reported on stderr and makes the exit status 2.  With the
+// environment variable SS_SEARCH_OUT set, the ids of each L are written to
+// $SS_SEARCH_OUT.L<L>.ids in the ground-truth file format ([u32 nq][u32 K]
+// then nq x K u32).
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -58,6 +66,10 @@ int main(int argc, char **argv)
         std::vector<uint32_t> ids((size_t)nq * K);
         std::vector<float> dists((size_t)nq * K);
         std::vector<ss_query_stats> st(nq);
+        const char *out_prefix = std::getenv("SS_SEARCH_OUT");
+        int status = 0;
+        ss_compat::visited_bits flags(num, 0);  // tests/search.cpp:71 (unused by the GPU)
+        std::vector<unsigned> one;
         for (int L : Ls) {
             engine.SearchBatch(queries.data(), nq, nullptr, nullptr, K, L, K, ids.data(), dists.data(), st.data());
             auto s = std::chrono::high_resolution_clock::now();
@@ -87,10 +99,25 @@ int main(int argc, char **argv)
             std::printf("%d,%g,%g,%g,%g,%g,%g,%g,%g\n", L, nq / secs, secs * 1e3 / nq, ndc / nq, avg,
                         recalls[(size_t)(nq * 0.05)], recalls[(size_t)(nq * 0.01)], service[(size_t)(nq * 0.5)],
                         service[std::min<size_t>(nq - 1, (size_t)(nq * 0.99))]);
+            const uint32_t n1 = std::min<uint32_t>(nq, 200);
+            uint32_t same = 0;
+            for (uint32_t i = 0; i < n1; i++) {
+                engine.SearchArraySimulation(queries.data() + (size_t)i * dim, i, K, L, flags, one);
+                same += std::equal(one.begin(), one.end(), ids.begin() + (size_t)i * K) ? 1u : 0u;
+            }
+            std::fprintf(stderr, "L=%d: %u of %u per-query SearchArraySimulation calls equal the batch\n", L, same,
+                         n1);
+            if (same != n1)
+                status = 2;
+            if (out_prefix) {
+                const std::string path = std::string(out_prefix) + ".L" + std::to_string(L) + ".ids";
+                if (ss_gt_write(path.c_str(), ids.data(), nq, (uint32_t)K) != SS_OK)
+                    throw std::runtime_error(ss_last_error());
+            }
         }
+        return status;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;
     }
-    return 0;
 }

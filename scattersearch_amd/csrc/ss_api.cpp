@@ -16,8 +16,34 @@ size_t bruteforce_lds_bytes(uint32_t dim_pad);
 uint32_t bruteforce_kmax();
 
 static thread_local std::string g_error;
+static thread_local const char *g_error_literal = nullptr;
 
-void set_error(const std::string &msg) { g_error = msg; }
+void set_error(const std::string &msg)
+{
+    g_error = msg;
+    g_error_literal = nullptr;
+}
+
+void set_error(const char *literal) { g_error_literal = literal; }
+
+// Distances of NaN or infinite coordinates have no order, so such vectors
+// are refused where they enter through host memory.
+int check_finite_vectors(const char *who, const char *what, const float *v, size_t n, uint32_t dim)
+{
+    for (size_t r = 0; r < n; r++) {
+        const float *row = v + r * dim;
+        for (uint32_t j = 0; j < dim; j++) {
+            uint32_t b;
+            std::memcpy(&b, row + j, 4);
+            if ((b & 0x7F800000u) == 0x7F800000u) {
+                set_error(std::string(who) + ": " + what + " " + std::to_string(r) +
+                          " has a NaN or infinite coordinate");
+                return SS_EINVAL;
+            }
+        }
+    }
+    return SS_OK;
+}
 
 }  // namespace ss
 
@@ -108,7 +134,7 @@ struct ss_index {
 
 extern "C" {
 
-const char *ss_last_error(void) { return ss::g_error.c_str(); }
+const char *ss_last_error(void) { return ss::g_error_literal ? ss::g_error_literal : ss::g_error.c_str(); }
 
 uint32_t ss_dim_padded(uint32_t dim) { return (dim + 7u) & ~7u; }
 
@@ -137,6 +163,11 @@ static int init_device(ss_index *ix)
     if ((e = hipMalloc(reinterpret_cast<void **>(&ix->d_queue), 64)) != hipSuccess)
         return hip_fail(e, "hipMalloc(queue)");
     return SS_OK;
+}
+
+static int check_finite(const char *who, const char *what, const float *v, size_t n, uint32_t dim)
+{
+    return ss::check_finite_vectors(who, what, v, n, dim);
 }
 
 static int validate_create(uint32_t dim, uint32_t num, const void *base, int metric, ss_index **out)
@@ -196,6 +227,8 @@ int ss_create(uint32_t dim, uint32_t num, const float *base, int metric, ss_inde
 {
     int rc = validate_create(dim, num, base, metric, out);
     if (rc)
+        return rc;
+    if ((rc = check_finite("ss_create", "base vector", base, num, dim)))
         return rc;
     ss_index *ix = nullptr;
     if ((rc = create_common(dim, num, metric, out, &ix)))
@@ -315,7 +348,12 @@ int ss_set_graph(ss_index *ix, const uint32_t *adj, uint32_t r_pad, uint32_t ep)
         set_error("ss_set_graph: entry point out of range");
         return SS_EINVAL;
     }
-    // Validate: ids < num, rows prefix-filled; drop repeated ids in a row.
+    // Validate: ids < num, rows prefix-filled.  Repeated ids inside a row
+    // are dropped (the reference's visited set skips a repeat wherever a
+    // row is expanded, src/annsearch.cpp:229-231, and the device tests a
+    // whole row at once), except in the entry point's row: the reference
+    // seeds its pool with the first min(L, deg) entries of that row as they
+    // are, repeats included (src/annsearch.cpp:187-214).
     std::vector<uint32_t> clean((size_t)ix->num * r_pad, SS_NONE);
     for (uint32_t n = 0; n < ix->num; n++) {
         const uint32_t *row = adj + (size_t)n * r_pad;
@@ -338,7 +376,7 @@ int ss_set_graph(ss_index *ix, const uint32_t *adj, uint32_t r_pad, uint32_t ep)
                 return SS_EINVAL;
             }
             bool dup = false;
-            for (uint32_t x = 0; x < w; x++)
+            for (uint32_t x = 0; x < w && n != ep; x++)
                 dup |= dst[x] == id;
             if (!dup)
                 dst[w++] = id;
@@ -470,10 +508,11 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_EINVAL;
     }
     const uint32_t wq = p && p->waves_per_query ? p->waves_per_query : 1;
-    // 8 wavefronts per query are refused: that configuration was followed by a
-    // GPU fault in a batched run whose cause has not been found.
-    if (wq < 1 || wq > ss::kMaxWavesPerQueryEnabled || (wq > 1 && ew != 1)) {
-        set_error("waves_per_query must be 1..4, and expand_width 1 when it is above 1");
+    // Up to eight wavefronts per query (the reference's default of eight
+    // threads, scripts/parallel_search.sh:3); the LDS control block of the
+    // several-wavefronts search holds eight.
+    if (wq < 1 || wq > ss::kMaxWavesPerQueryHost || (wq > 1 && ew != 1)) {
+        set_error("waves_per_query must be 1..8, and expand_width 1 when it is above 1");
         return SS_EINVAL;
     }
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
@@ -634,7 +673,7 @@ static int prepare_order(ss_index *ix, uint32_t nq, hipStream_t st)
 static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
                       uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
                       uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st,
-                      uint32_t *d_path = nullptr, uint32_t path_cap = 0, uint32_t *d_queue = nullptr)
+                      uint32_t *d_path = nullptr, uint32_t path_cap = 0)
 {
     if (!ix->has_graph) {
         set_error("search: no graph loaded");
@@ -709,29 +748,29 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         }
         return SS_OK;
     }
-    a.queue = d_queue ? d_queue : ix->d_queue;
-    const bool locality = !d_path && !d_queue && a.waves_per_query == 1 &&
+    a.queue = ix->d_queue;
+    const bool locality = !d_path && a.waves_per_query == 1 &&
                           (p->query_order == SS_ORDER_LOCALITY ||
                            (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH));
     ss::OrderArgs oa;
     std::memset(&oa, 0, sizeof(oa));
     if (locality) {
-        if ((rc = prepare_order(ix, nq, st)) != SS_OK)
-            return rc;
         oa.queries = d_queries;
-        oa.pivots = ix->d_pivots;
         oa.nq = nq;
         oa.dim_pad = ix->dim_pad;
+        oa.queue = a.queue;
+        if ((rc = prepare_order(ix, nq, st)) != SS_OK)
+            return rc;
+        oa.pivots = ix->d_pivots;
         oa.key_dims = ix->pivot_dims;
         oa.keys = ix->order.ptr;
         oa.order = ix->order.ptr + ix->order_n;
-        oa.queue = a.queue;
         a.order = oa.order;
     }
     auto reset = [&]() {
         return locality ? ss::launch_order(oa, (uint32_t)ix->num_cus, st) : ss::launch_queue_reset(a.queue, st);
     };
-    const bool use_graph = p && p->use_graph && !d_path && !d_queue;
+    const bool use_graph = p && p->use_graph && !d_path;
     if (use_graph) {
         const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
                           ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
@@ -824,6 +863,9 @@ static int search_host(ss_index *ix, const float *queries, uint32_t nq, const ui
         set_error("ss_search: K larger than k_stride");
         return SS_EINVAL;
     }
+    int frc = check_finite("ss_search", "query", queries, nq, ix->dim);
+    if (frc)
+        return frc;
     std::lock_guard<std::mutex> lk(ix->mu);
     hipError_t e;
     if ((e = ix->q.reserve((size_t)nq * ix->dim_pad)) != hipSuccess || (e = ix->kl.reserve((size_t)nq * 2)) != hipSuccess ||
@@ -993,6 +1035,9 @@ int ss_bruteforce(ss_index *ix, const float *queries, uint32_t nq, uint32_t k, c
         }
         return SS_OK;
     }
+    int frc = check_finite("ss_bruteforce", "query", queries, nq, ix->dim);
+    if (frc)
+        return frc;
     std::lock_guard<std::mutex> lk(ix->mu);
     hipError_t e;
     if ((e = ix->q.reserve((size_t)nq * ix->dim_pad)) != hipSuccess)

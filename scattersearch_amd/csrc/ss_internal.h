@@ -131,7 +131,6 @@ size_t search_lds_bytes_per_wave(uint32_t dim_pad, uint32_t l_max, uint32_t hash
 size_t scatter_lds_bytes_per_block(uint32_t dim_pad, uint32_t l_max, uint32_t hash_slots,
                                    uint32_t waves_per_query);
 constexpr uint32_t kMaxWavesPerQueryHost = 8;  // LDS control arrays are sized for this
-constexpr uint32_t kMaxWavesPerQueryEnabled = 4;  // accepted by the host (see choose_shape)
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks,
@@ -160,6 +159,14 @@ bool dim_supported(uint32_t dim_pad);
 hipError_t init_kernel_attributes(size_t max_lds);
 
 void set_error(const std::string &msg);
+// Records a message of static storage (a string literal) without copying it:
+// error paths that must not allocate use this form.
+void set_error(const char *literal);
+
+
+// SS_EINVAL (with a message naming the first bad row) if any of the n rows
+// [n][dim] of v holds a NaN or infinite value.
+int check_finite_vectors(const char *who, const char *what, const float *v, size_t n, uint32_t dim);
 
 }  // namespace ss
 

@@ -616,11 +616,15 @@ __device__ __forceinline__ void seed_pool(const SearchArgs &a, const WaveLds &w,
     wave_sync();
     candidate_distances<NBLK, TAILI, METRIC>(a, w, t, nblk_rt, tail_rt);
     wave_sync();
+    // Rank by (order-preserving key of the distance, row position): a strict
+    // total order, so the first t lanes get distinct ranks even for distances
+    // that do not compare (NaN).
     const float di = lane < t ? w.cand_d[lane] : __builtin_inff();
+    const uint32_t ki = dist_key<METRIC>(di);
     uint32_t rank = 0;
     for (uint32_t j = 0; j < t; j++) {
-        const float dj = readlane_f(di, j);
-        rank += (dj < di || (dj == di && j < lane)) ? 1u : 0u;
+        const uint32_t kj = readlane_u(ki, j);
+        rank += (kj < ki || (kj == ki && j < lane)) ? 1u : 0u;
     }
     wave_sync();
     if (lane < t) {
@@ -1724,10 +1728,14 @@ __device__ void steal_entries(const SearchArgs &a, const ScatterLds &s, uint32_t
         }
         const float cd = lane < nc && !dup ? w.cand_d[lane] : __builtin_inff();
         const uint32_t nkeep = nc - (uint32_t)__popcll(ballot(lane < nc && dup));
+        // Rank by (repeat, order-preserving key of the distance, claim
+        // order): a strict total order, so the nkeep kept entries fill ranks
+        // 0..nkeep-1 whatever their distances (+inf or NaN included).
+        const uint32_t dk = dup ? 1u : 0u, key = dist_key<METRIC>(cd);
         uint32_t rank = 0;
         for (uint32_t j = 0; j < nc; j++) {
-            const float dj = readlane_f(cd, j);
-            rank += (dj < cd || (dj == cd && j < lane)) ? 1u : 0u;
+            const uint32_t dj = readlane_u(dk, j), kj = readlane_u(key, j);
+            rank += (dj < dk || (dj == dk && (kj < key || (kj == key && j < lane)))) ? 1u : 0u;
         }
         wave_sync();
         if (lane < nc) {

@@ -373,6 +373,8 @@ int ss_submit(ss_server *s, const float *query, uint32_t K, uint32_t L, uint64_t
         set_error("ss_submit: K and L exceed the server's budget (1 <= K <= L, K <= k_max, L <= l_max)");
         return SS_EINVAL;
     }
+    if (int frc = ss::check_finite_vectors("ss_submit", "query", query, 1, s->v.dim))
+        return frc;
     const uint64_t idx = s->submitted.load(std::memory_order_relaxed);
     if (idx - s->completed.load(std::memory_order_acquire) >= s->S) {
         set_error("ss_submit: ring full");

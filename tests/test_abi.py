@@ -85,3 +85,16 @@ def test_status_codes_match_the_header():
     assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
     assert ctypes.sizeof(_lib.SearchParams) == 32 and _lib.SearchParams.query_order.offset == 28
     assert "uint32_t query_order;" in hdr
+
+
+def test_non_finite_base_is_refused():
+    """NaN and infinite coordinates have no distance order; ss_create refuses
+    them before it touches the device."""
+    from scattersearch_amd._lib import lib, SS_EINVAL
+    h = ctypes.c_void_p()
+    for bad in (np.nan, np.inf, -np.inf):
+        base = np.zeros((4, 8), dtype=np.float32)
+        base[2, 5] = bad
+        p = base.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        assert lib.ss_create(8, 4, p, 0, ctypes.byref(h)) == SS_EINVAL
+        assert b"base vector 2" in lib.ss_last_error()

@@ -184,6 +184,19 @@ def test_forgetful_visited_set_changes_only_distance_count(seed):
         assert m[2] >= o[2]
 
 
+def test_repeated_entry_row_ids_seed_the_pool_twice():
+    """The reference seeds the pool with the entry point's row as it is
+    (src/annsearch.cpp:187-214): a repeated id takes two pool entries and is
+    expanded twice (the second expansion finds every neighbour visited)."""
+    base = np.array([[0.0], [1.0], [3.0], [10.0]], dtype=np.float32).repeat(8, axis=1)
+    adj = np.full((4, 4), NONE, dtype=np.uint32)
+    adj[3, :3] = [1, 1, 2]
+    q = np.zeros(8, dtype=np.float32)
+    ids, ds, ndc, hops = oracle.search(base, adj, 3, q, 3, 3)
+    assert ids.tolist() == [1, 1, 2]
+    assert ndc == 3 and hops == 3
+
+
 def test_complete_graph_search_is_exact():
     """On a complete graph with L >= N the pool ends up holding every point,
     so the search returns the exact top-K."""

@@ -32,9 +32,7 @@ def _check_entries(base, q, ids, ds, K, metric=0):
         assert (ids[i, k:] == 0xFFFFFFFF).all()
 
 
-@pytest.mark.parametrize("W", [2, 4, pytest.param(8, marks=pytest.mark.skip(
-    reason="8 wavefronts per query are refused by the library: a batched run of that configuration "
-           "was followed by a GPU fault whose cause has not been found"))])
+@pytest.mark.parametrize("W", [2, 4, 8])
 def test_scatter_recall(c0, W):
     import oracle
     from scattersearch_amd import metrics
@@ -73,7 +71,7 @@ def test_scatter_repeated_launches_stay_valid(small):
         _check_entries(small["base"], q, ids, ds, 10)
 
 
-@pytest.mark.parametrize("W", [2, 4])
+@pytest.mark.parametrize("W", [2, 4, 8])
 def test_scatter_long_batches_stay_in_bounds(c0, W):
     """Regression: a wavefront whose entries were being stolen used to read a
     pool id a second time after the ballot that found it unexplored and could
@@ -109,7 +107,7 @@ def test_scatter_rejects_bad_parameters(small):
     from scattersearch_amd._lib import SS_EINVAL
     ix = _index(small)
     q = small["queries"][:4]
-    for kw in ({"waves_per_query": 9}, {"waves_per_query": 8}, {"waves_per_query": 2, "expand_width": 2}):
+    for kw in ({"waves_per_query": 9}, {"waves_per_query": 2, "expand_width": 2}):
         with pytest.raises(SSError) as e:
             ix.search(q, 10, 48, **kw)
         assert e.value.code == SS_EINVAL

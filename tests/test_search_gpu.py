@@ -55,6 +55,31 @@ def test_c0_pool_sizes(c0, L):
     _assert_same(gpu, ora, check_ndc=gpu[2]["lossy"].sum() == 0, what=f"L={L}")
 
 
+def test_entry_row_repeats_kept_other_rows_deduplicated(small):
+    """Repeats in the entry point's row seed the pool as in the reference
+    (two entries, src/annsearch.cpp:187-214); repeats elsewhere are dropped at
+    load (the reference's visited set skips them).  Results, distances, NDC
+    and hops equal the oracle on the graph as given."""
+    import oracle
+    base, q = small["base"], small["queries"]
+    adj = small["adj"].copy()
+    ep = small["ep"]
+    deg = int((adj[ep] != NONE).sum())
+    assert deg >= 4
+    adj[ep, 1] = adj[ep, 0]  # the entry row names its first neighbour twice
+    other = (ep + 1) % adj.shape[0]
+    dother = int((adj[other] != NONE).sum())
+    adj[other, dother - 1] = adj[other, 0]
+    ix = _index(base, adj, ep)
+    got, _ = ix.get_graph()
+    assert got[ep].tolist() == adj[ep].tolist()
+    assert (got[other] != NONE).sum() == dother - 1
+    for L in (deg, 48):
+        gpu = ix.search(q, 10, L, visited_slots=8192)
+        ora = oracle.search_batch(base, adj, ep, q, 10, L)
+        _assert_same(gpu, ora, what=f"repeated entry-row id, L={L}")
+
+
 def test_per_query_launch_equals_batched(small):
     ix = _index(small["base"], small["adj"], small["ep"])
     from scattersearch_amd import SS_MODE_PER_QUERY

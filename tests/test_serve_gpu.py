@@ -17,13 +17,6 @@ def _index(d):
     return ix
 
 
-@pytest.mark.skip(reason="an earlier resident server (worker wavefronts waiting for a dispatcher wavefront "
-                         "to publish their slot) hung on the GPU (no output for 180 s) and was removed; the "
-                         "current serving kernel never waits on another wavefront and is tested below")
-def test_replay_results_equal_alone(small):
-    pass
-
-
 def test_serving_results_equal_alone(small):
     import oracle
     from scattersearch_amd import serve
@@ -43,6 +36,37 @@ def test_serving_results_equal_alone(small):
         assert np.array_equal(ds[i, :k].view(np.uint32), o_d.view(np.uint32))
         assert (ids[i, k:] == 0xFFFFFFFF).all()
         assert st["hops"][i] == o_hops and st["ndc"][i] == o_ndc
+
+
+def test_serving_results_equal_alone_at_the_serving_mix(small):
+    """The serving workload's request mix (K = 10 for 80% of the requests and
+    100 for the rest, L uniform in max(64, K)..512) through activations sized
+    for k_max = 100 and l_max = 512 (pools of up to eight 64-entry chunks):
+    every result equals the oracle's search of that request alone, ids,
+    distance bits and hops, and the distance count wherever the visited hash
+    forgot nothing."""
+    import oracle
+    from scattersearch_amd import serve
+    ix = _index(small)
+    rng = np.random.default_rng(17)
+    q = np.concatenate([small["queries"]] * 6)
+    nq = q.shape[0]
+    K = np.where(rng.random(nq) < 0.8, 10, 100).astype(np.uint32)
+    L = rng.integers(np.maximum(64, K), 513).astype(np.uint32)
+    arr = serve.poisson_arrivals(nq, 50000, seed=3)
+    print(f"serving mix: {nq} requests, K in {{10, 100}}, L {L.min()}..{L.max()}", flush=True)
+    ids, ds, lat, st = serve.replay(ix, q, K, L, arr, k_max=100, l_max=512)
+    assert (lat > 0).all()
+    assert (st["status"] == 0).all()
+    o_ids, o_d, o_ndc, o_hops = oracle.search_batch(small["base"], small["adj"], small["ep"], q, K, L, k_stride=100)
+    for i in range(nq):
+        k = int(K[i])
+        assert np.array_equal(ids[i, :k], o_ids[i, :k]), f"request {i} K={k} L={L[i]}"
+        assert np.array_equal(ds[i, :k].view(np.uint32), o_d[i, :k].view(np.uint32)), f"request {i}"
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+        assert st["hops"][i] == o_hops[i], f"request {i}"
+        if st["lossy"][i] == 0:
+            assert st["ndc"][i] == o_ndc[i], f"request {i}"
 
 
 def test_submit_and_poll(small):
