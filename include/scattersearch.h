@@ -182,6 +182,26 @@ int ss_search_device(ss_index *index, const float *d_queries, uint32_t nq,
                      uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats,
                      void *stream);
 
+/* ss_search_device for callers that cannot allocate (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, results and stream
+ * semantics as ss_search_device, but the call allocates nothing, on the
+ * device or the host: the work-queue counter and the locality order live in
+ * a scratch array of about 1 MiB that the library holds on each GPU for the
+ * life of the process, and nothing is cached per index (the order's pivots
+ * are made again in each call; params->use_graph must be 0, since the
+ * index's graph cache allocates).  A batched call takes at most
+ * SS_NOALLOC_MAX_QUERIES queries (SS_EINVAL above).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU, on any index or
+ * stream, race and must be ordered by the caller.  Each call clears what it
+ * needs with kernels (no memset nodes), so it can be captured in a graph. */
+#define SS_NOALLOC_MAX_QUERIES 131072u
+int ss_search_device_noalloc(ss_index *index, const float *d_queries, uint32_t nq,
+                             const uint32_t *d_K, const uint32_t *d_L, uint32_t k_max,
+                             uint32_t l_max, uint32_t k_stride, const ss_search_params *params,
+                             uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats,
+                             void *stream);
+
 /* Exact k nearest base points per query by (distance, id), distances in
  * the same arithmetic as the search (ground truth; load_groundtruth files,
  * src/util.cpp:72-93, hold its ids).  exclude (optional, host, [nq]) is an

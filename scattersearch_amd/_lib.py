@@ -33,6 +33,7 @@ SS_ORDER_AUTO = 0
 SS_ORDER_INDEX = 1
 SS_ORDER_LOCALITY = 2
 SS_ORDER_MIN_BATCH = 4096
+SS_NOALLOC_MAX_QUERIES = 131072
 
 
 class SearchParams(ctypes.Structure):
@@ -124,6 +125,8 @@ _SIGS = {
                               u32p, f32p, ctypes.POINTER(QueryStats), u32p]),
     "ss_search_device": (cint, [vp, vp, u32, vp, vp, u32, u32, u32,
                                 ctypes.POINTER(SearchParams), vp, vp, vp, vp]),
+    "ss_search_device_noalloc": (cint, [vp, vp, u32, vp, vp, u32, u32, u32,
+                                        ctypes.POINTER(SearchParams), vp, vp, vp, vp]),
     "ss_bruteforce": (cint, [vp, f32p, u32, u32, u32p, u32p, f32p]),
     "ss_bruteforce_device": (cint, [vp, vp, u32, u32, cint, u32, vp, vp, vp]),
     "ss_knn_graph": (cint, [vp, u32, u32p, f32p]),

@@ -104,6 +104,8 @@ struct ss_index {
     uint32_t r_pad = 0, ep = 0;
     bool has_graph = false;
     uint32_t *d_queue = nullptr;
+    // This device's scratch array (ss_search_device_noalloc), found at set-up.
+    uint32_t *d_scratch = nullptr;
     // Locality order of batched searches: pivots [pivot_dims][kOrderPivots]
     // (made on first use), keys and order of up to order_n queries.
     float *d_pivots = nullptr;
@@ -162,6 +164,9 @@ static int init_device(ss_index *ix)
         return hip_fail(e, "hipStreamCreate");
     if ((e = hipMalloc(reinterpret_cast<void **>(&ix->d_queue), 64)) != hipSuccess)
         return hip_fail(e, "hipMalloc(queue)");
+    // the current device is ix->device here
+    if ((e = ss::scratch_address(ix->device, &ix->d_scratch)) != hipSuccess)
+        return hip_fail(e, "hipGetSymbolAddress(scratch)");
     return SS_OK;
 }
 
@@ -673,8 +678,11 @@ static int prepare_order(ss_index *ix, uint32_t nq, hipStream_t st)
 static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K, const uint32_t *d_L,
                       uint32_t k_max, uint32_t l_max, uint32_t k_stride, const ss_search_params *p,
                       uint32_t *d_out_ids, float *d_out_dists, ss_query_stats *d_stats, hipStream_t st,
-                      uint32_t *d_path = nullptr, uint32_t path_cap = 0)
+                      uint32_t *d_path = nullptr, uint32_t path_cap = 0, uint32_t *scratch = nullptr)
 {
+    // scratch (ss_search_device_noalloc): the device's scratch array holds
+    // the work-queue counter and the locality order, so that nothing is
+    // allocated; the caller has checked nq and use_graph.
     if (!ix->has_graph) {
         set_error("search: no graph loaded");
         return SS_ESTATE;
@@ -748,7 +756,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         }
         return SS_OK;
     }
-    a.queue = ix->d_queue;
+    a.queue = scratch ? scratch : ix->d_queue;
     const bool locality = !d_path && a.waves_per_query == 1 &&
                           (p->query_order == SS_ORDER_LOCALITY ||
                            (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH));
@@ -759,18 +767,30 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         oa.nq = nq;
         oa.dim_pad = ix->dim_pad;
         oa.queue = a.queue;
-        if ((rc = prepare_order(ix, nq, st)) != SS_OK)
-            return rc;
-        oa.pivots = ix->d_pivots;
-        oa.key_dims = ix->pivot_dims;
-        oa.keys = ix->order.ptr;
-        oa.order = ix->order.ptr + ix->order_n;
+        if (scratch) {
+            // pivots of this index, made again each call: every index on the
+            // device shares the array
+            oa.key_dims = std::min<uint32_t>(ix->dim_pad, ss::kOrderKeyDims);
+            float *piv = reinterpret_cast<float *>(scratch + ss::kScratchPivotOff);
+            if ((e = ss::launch_order_pivots(ix->d_base, ix->dim_pad, ix->num, oa.key_dims, piv, st)) != hipSuccess)
+                return hip_fail(e, "order pivots");
+            oa.pivots = piv;
+            oa.keys = scratch + ss::kScratchKeysOff;
+            oa.order = scratch + ss::kScratchOrderOff;
+        } else {
+            if ((rc = prepare_order(ix, nq, st)) != SS_OK)
+                return rc;
+            oa.pivots = ix->d_pivots;
+            oa.key_dims = ix->pivot_dims;
+            oa.keys = ix->order.ptr;
+            oa.order = ix->order.ptr + ix->order_n;
+        }
         a.order = oa.order;
     }
     auto reset = [&]() {
         return locality ? ss::launch_order(oa, (uint32_t)ix->num_cus, st) : ss::launch_queue_reset(a.queue, st);
     };
-    const bool use_graph = p && p->use_graph && !d_path;
+    const bool use_graph = p && p->use_graph && !d_path && !scratch;
     if (use_graph) {
         const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
                           ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
@@ -827,6 +847,34 @@ int ss_search_device(ss_index *ix, const float *d_queries, uint32_t nq, const ui
     }
     return run_search(ix, d_queries, nq, d_K, d_L, k_max, l_max, k_stride, params, d_out_ids, d_out_dists, d_stats,
                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int ss_search_device_noalloc(ss_index *ix, const float *d_queries, uint32_t nq, const uint32_t *d_K,
+                             const uint32_t *d_L, uint32_t k_max, uint32_t l_max, uint32_t k_stride,
+                             const ss_search_params *params, uint32_t *d_out_ids, float *d_out_dists,
+                             ss_query_stats *d_stats, void *stream)
+{
+    if (!ix) {
+        ss::set_error("ss_search_device_noalloc: NULL index");
+        return SS_EINVAL;
+    }
+    const uint32_t mode = params ? params->mode : SS_MODE_BATCHED;
+    if (mode == SS_MODE_BATCHED && nq > SS_NOALLOC_MAX_QUERIES) {
+        ss::set_error("ss_search_device_noalloc: more than SS_NOALLOC_MAX_QUERIES (131072) queries in one "
+                      "batched call; split the batch");
+        return SS_EINVAL;
+    }
+    if (params && params->use_graph) {
+        ss::set_error("ss_search_device_noalloc: use_graph needs the index's own graph cache, which allocates; "
+                      "capture the call in a graph of your own instead");
+        return SS_EINVAL;
+    }
+    if (!ix->d_scratch) {
+        ss::set_error("ss_search_device_noalloc: the device's scratch array was not found at set-up");
+        return SS_EDEVICE;
+    }
+    return run_search(ix, d_queries, nq, d_K, d_L, k_max, l_max, k_stride, params, d_out_ids, d_out_dists, d_stats,
+                      reinterpret_cast<hipStream_t>(stream), nullptr, 0, ix->d_scratch);
 }
 
 static int search_host(ss_index *ix, const float *queries, uint32_t nq, const uint32_t *K, const uint32_t *L,

@@ -163,6 +163,23 @@ void set_error(const std::string &msg);
 // error paths that must not allocate use this form.
 void set_error(const char *literal);
 
+// Scratch memory of the search that allocates nothing (ss_search_device_noalloc):
+// a file-scope device array of the kernel module, u32 words laid out as
+//   [0, kScratchQueueWords)            work-queue counter (word 0)
+//   [kScratchPivotOff, +pivot words)   locality-order pivots
+//   [kScratchKeysOff, +max queries)    locality-order keys
+//   [kScratchOrderOff, +max queries)   locality order
+// Each region starts on a 256-byte boundary.
+constexpr uint32_t kScratchMaxQueries = SS_NOALLOC_MAX_QUERIES;
+constexpr uint32_t kScratchQueueWords = 64;
+constexpr uint32_t kScratchPivotOff = kScratchQueueWords;
+constexpr uint32_t kScratchPivotWords = kOrderKeyDims * kOrderPivots;
+constexpr uint32_t kScratchKeysOff = kScratchPivotOff + kScratchPivotWords;
+constexpr uint32_t kScratchOrderOff = kScratchKeysOff + kScratchMaxQueries;
+constexpr uint32_t kScratchWords = kScratchOrderOff + kScratchMaxQueries;
+// The device address of the scratch array on the current device (looked up
+// once per device with hipGetSymbolAddress and kept).
+hipError_t scratch_address(int device, uint32_t **out);
 
 // SS_EINVAL (with a message naming the first bad row) if any of the n rows
 // [n][dim] of v holds a NaN or infinite value.

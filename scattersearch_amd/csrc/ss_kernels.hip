@@ -19,6 +19,8 @@
 
 #include "ss_internal.h"
 
+#include <mutex>
+
 namespace ss {
 namespace {
 
@@ -2732,5 +2734,32 @@ hipError_t init_kernel_attributes(size_t max_lds)
 }
 
 bool dim_supported(uint32_t dim_pad) { return dim_pad >= 8 && dim_pad % 8 == 0 && dim_pad <= 4096; }
+
+// Scratch memory of the search that allocates nothing: one copy per GPU,
+// held for the life of the process (about 1 MiB; layout in ss_internal.h).
+// In host code the name is only a stand-in for the symbol; kernels get the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+hipError_t scratch_address(int device, uint32_t **out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static uint32_t *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices || !out)
+        return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        if (!p || (reinterpret_cast<uintptr_t>(p) & 255u))
+            return hipErrorInvalidSymbol;
+        addr[device] = static_cast<uint32_t *>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
 
 }  // namespace ss
