@@ -37,12 +37,37 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def host_threads():
+def host_cpus():
+    """(threads the CPU baseline uses, CPUs this process may run on, CPUs the
+    OS reports).  The baseline uses every CPU it may run on, at most 16: a
+    one-GPU box's share of its host is 16 CPUs, whatever nproc reports."""
+    n_os = os.cpu_count() or 1
     try:
-        n = len(os.sched_getaffinity(0))
+        n_aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        n_aff = n_os
+    return max(1, min(n_aff, 16)), n_aff, n_os
+
+
+def host_threads():
+    return host_cpus()[0]
+
+
+def cpu_model():
+    info = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                info[k] = v
+            if k == "flags":
+                break
+    except OSError:
+        pass
+    name = info.get("model name", "unknown")
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}, {info.get('cpu MHz', '?')} MHz)")
 
 
 def main():
@@ -74,7 +99,20 @@ def main():
                     help="also time hipGraph-replayed launches of these batch sizes; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=0.5,
                     help="length of the arrival stream at each serving rate (at least 20,000 requests)")
+    ap.add_argument("--gt-queries", type=int, default=-1,
+                    help="queries with exact ground truth (recall is measured on these); -1 = all, "
+                         "or the first 10,000 on the 10M workload")
     args = ap.parse_args()
+    if args.workload == "c3":
+        # the 10M x 96 workload: L in {64, 128} as configured, batch sizes up to the
+        # whole 100,000-query batch; its graph is not the serving workload's
+        if args.gt_queries < 0:
+            args.gt_queries = 10000
+        if "--serve-rates" not in sys.argv:
+            args.serve_rates = ""
+        if "--batch-sizes" not in sys.argv:
+            args.batch_sizes = "1,64,1024,10000,100000"
+        args.no_variants = True
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the query path "
@@ -161,8 +199,12 @@ def main():
         return
 
     t0 = time.time()
-    gt, _ = ix.bruteforce(queries, K)
-    log(f"ground truth in {time.time() - t0:.1f}s")
+    n_gt = nq if args.gt_queries < 0 else min(nq, args.gt_queries)
+    gt, _ = ix.bruteforce(queries[:n_gt], K)
+    log(f"ground truth of {n_gt} queries in {time.time() - t0:.1f}s")
+
+    def recall_of(ids):
+        return metrics.recall(ids[:n_gt], gt, K)
 
     def timed(L, steps, warmup, **kw):
         for _ in range(warmup):
@@ -187,7 +229,7 @@ def main():
         wall, kern_ms = timed(L, 3, 1)
         ids = d_ids.cpu().numpy().view(np.uint32)
         st = stats_host()
-        rec = metrics.recall(ids, gt, K)
+        rec = recall_of(ids)
         row = dict(L=L, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
                    mean_ndc=round(float(st["ndc"].mean()), 1), mean_hops=round(float(st["hops"].mean()), 1),
                    lossy=int(st["lossy"].sum()))
@@ -198,13 +240,13 @@ def main():
     if chosen is None:
         chosen = L_list[-1]
         log(f"no L reached recall {args.recall_target}; reporting the largest L")
-    elif not args.L:
+    elif not args.L and args.workload == "c1":
         # refine between the last L below the target and the first above it
         lo = max([r["L"] for r in sweep if r["L"] < chosen] or [K])
         for Lr in range(((lo + 8) // 8) * 8, chosen, 8):
             wall, kern_ms = timed(Lr, 3, 1)
             ids = d_ids.cpu().numpy().view(np.uint32)
-            rec = metrics.recall(ids, gt, K)
+            rec = recall_of(ids)
             st = stats_host()
             row = dict(L=Lr, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
                        mean_ndc=round(float(st["ndc"].mean()), 1), mean_hops=round(float(st["hops"].mean()), 1),
@@ -227,7 +269,7 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     ids = d_ids.cpu().numpy().view(np.uint32)
     st = stats_host()
-    rec = metrics.recall(ids, gt, K)
+    rec = recall_of(ids)
     khz = ix.wall_clock_khz()
     service_ms = (st["t_end"].astype(np.float64) - st["t_start"]) / khz
     done_ms = (st["t_end"].astype(np.float64) - st["t_start"].min()) / khz
@@ -296,7 +338,7 @@ def main():
                     rows.append(dict(L=Lv, error=str(exc)))
                     continue
                 st_v = stats_host()
-                rows.append(dict(L=Lv, recall=round(metrics.recall(d_ids.cpu().numpy().view(np.uint32), gt, K), 4),
+                rows.append(dict(L=Lv, recall=round(recall_of(d_ids.cpu().numpy().view(np.uint32)), 4),
                                  qps=round(nq * 3 / wall_v, 1), kernel_ms=round(kern_v, 3),
                                  mean_ndc=round(float(st_v["ndc"].mean()), 1)))
             ok = [r for r in rows if r.get("recall", 0) >= args.recall_target]
@@ -515,21 +557,14 @@ def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):
     queries, each about `seconds` of CPU wall time."""
     import oracle
     from scattersearch_amd import metrics
-    T = host_threads()
-    model = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    T, n_aff, n_os = host_cpus()
+    model = cpu_model()
     # calibrate the sample size on 20 queries
     n0 = min(20, queries.shape[0])
     t = time.perf_counter()
     oracle.scatter_search(base, adj, ep, queries[:n0], K, L, T, metric)
     per_q = (time.perf_counter() - t) / n0
-    n = int(min(queries.shape[0], max(n0, seconds / max(per_q, 1e-6))))
+    n = int(min(queries.shape[0], gt.shape[0], max(n0, seconds / max(per_q, 1e-6))))
     t = time.perf_counter()
     ids, lat, ndc = oracle.scatter_search(base, adj, ep, queries[:n], K, L, T, metric)
     wall = time.perf_counter() - t
@@ -540,15 +575,14 @@ def cpu_baseline(base, adj, ep, queries, gt, K, L, metric, seconds):
     t = time.perf_counter()
     oracle.search_batch(base, adj, ep, queries[:n0], K, L, metric, nthreads=T)
     per_q = (time.perf_counter() - t) / n0
-    n2 = int(min(queries.shape[0], max(n0, seconds / max(per_q, 1e-6))))
+    n2 = int(min(queries.shape[0], gt.shape[0], max(n0, seconds / max(per_q, 1e-6))))
     t = time.perf_counter()
     sids = oracle.search_batch(base, adj, ep, queries[:n2], K, L, metric, nthreads=T)[0]
     ser_qps = n2 / (time.perf_counter() - t)
     log(f"cpu serial search x{T} threads: {ser_qps:.0f} QPS over {n2} queries")
     return {"value": round(et_qps, 1), "unit": "queries/s", "cores": T, "kind": "port",
-            "sample": f"first {n} of the same queries at L={L}, K={K}, one after another, "
-                      f"{T} intra-query threads (ScatterSearch with election at hop 50)",
-            "recall@10": round(et_rec, 4), "cpu": model,
+            "sample": f"first {n} queries, L={L}, K={K}, serial, {T} threads per query (ScatterSearch)",
+            "recall@10": round(et_rec, 4), "cpu": model, "cpus_allowed": n_aff, "cpus_reported": n_os,
             "serial_search_qps": round(ser_qps, 1),
             "serial_search_sample": f"first {n2} queries, {T} queries in parallel, private visited sets",
             "serial_search_recall@10": round(metrics.recall(sids, gt[:n2], K), 4)}

@@ -340,6 +340,16 @@ int ss_build_graph(const float *base, uint32_t num, uint32_t dim, int metric,
                    uint32_t R, float alpha, uint32_t ep, int nthreads, uint32_t *adj_out,
                    uint32_t *unreachable_fixed);
 
+/* ss_build_graph for the base of an index, with its two occlusion prunes
+ * (candidates, then each list united with its reverse edges) on the GPU;
+ * the connectivity pass stays on the host.  base is the index's base on
+ * the host ([num][dim], the vectors given to ss_create), used by the host
+ * steps; the metric is the index's.  knn_k + extra_k <= 512.  The graph is
+ * identical to ss_build_graph's on the same arguments. */
+int ss_build_graph_gpu(ss_index *index, const float *base, const uint32_t *knn, uint32_t knn_k,
+                       const uint32_t *extra, uint32_t extra_k, uint32_t R, float alpha,
+                       uint32_t ep, uint32_t *adj_out, uint32_t *unreachable_fixed);
+
 /* Number of nodes reachable from ep (breadth-first over adj). */
 int ss_graph_reachable(const uint32_t *adj, uint32_t num, uint32_t r_pad, uint32_t ep,
                        uint32_t *reachable);

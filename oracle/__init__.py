@@ -205,8 +205,9 @@ def cpu_graph(base, metric, R, knn_k, nav_L, alpha=1.0, passes=3, nthreads=0, lo
         ex = np.arange(base.shape[0], dtype=np.uint32)
         return bruteforce(base, base, k, metric, exclude=ex, nthreads=nthreads)[0]
 
-    def search_fn(adj, ep, L, path_cap):
+    def search_fn(adj, ep, L, path_cap, out_pool, out_path):
         r = search_batch(base, adj, ep, base, L, L, metric, nthreads, path_cap=path_cap)
-        return r[0], r[-1]
+        out_pool[:] = r[0]
+        out_path[:] = r[-1]
 
     return build_graph(base, metric, R, knn_k, nav_L, knn_fn, search_fn, alpha, passes, nthreads, log)

@@ -160,6 +160,7 @@ _SIGS = {
     "ss_build_graph": (cint, [f32p, u32, u32, cint, u32p, u32, u32p, u32, u32,
                               ctypes.c_float, u32, cint, u32p, u32p]),
     "ss_graph_reachable": (cint, [u32p, u32, u32, u32, u32p]),
+    "ss_build_graph_gpu": (cint, [vp, f32p, u32p, u32, u32p, u32, u32, ctypes.c_float, u32, u32p, u32p]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -3,11 +3,14 @@
 // (include/annsearch.h:37-84); every entry point returns a status instead of
 // exiting and records a message for ss_last_error().
 #include "ss_internal.h"
+#include "ss_graph_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 namespace ss {
@@ -359,33 +362,53 @@ int ss_set_graph(ss_index *ix, const uint32_t *adj, uint32_t r_pad, uint32_t ep)
     // whole row at once), except in the entry point's row: the reference
     // seeds its pool with the first min(L, deg) entries of that row as they
     // are, repeats included (src/annsearch.cpp:187-214).
+    // Rows are checked by several threads (large graphs); the first bad row
+    // is reported.
     std::vector<uint32_t> clean((size_t)ix->num * r_pad, SS_NONE);
-    for (uint32_t n = 0; n < ix->num; n++) {
-        const uint32_t *row = adj + (size_t)n * r_pad;
-        uint32_t *dst = clean.data() + (size_t)n * r_pad;
-        uint32_t w = 0;
-        bool ended = false;
-        for (uint32_t j = 0; j < r_pad; j++) {
-            const uint32_t id = row[j];
-            if (id == SS_NONE) {
-                ended = true;
-                continue;
+    const uint32_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint32_t per = (ix->num + nt - 1) / nt;
+    std::vector<uint32_t> bad_row(nt, SS_NONE), bad_id(nt, SS_NONE);
+    auto check_rows = [&](uint32_t t) {
+        const uint32_t n1 = std::min(ix->num, (t + 1) * per);
+        for (uint32_t n = t * per; n < n1; n++) {
+            const uint32_t *row = adj + (size_t)n * r_pad;
+            uint32_t *dst = clean.data() + (size_t)n * r_pad;
+            uint32_t w = 0;
+            bool ended = false;
+            for (uint32_t j = 0; j < r_pad; j++) {
+                const uint32_t id = row[j];
+                if (id == SS_NONE) {
+                    ended = true;
+                    continue;
+                }
+                if (ended || id >= ix->num) {
+                    bad_row[t] = n;
+                    bad_id[t] = ended ? SS_NONE : id;
+                    return;
+                }
+                bool dup = false;
+                for (uint32_t x = 0; x < w && n != ep; x++)
+                    dup |= dst[x] == id;
+                if (!dup)
+                    dst[w++] = id;
             }
-            if (ended) {
-                set_error("ss_set_graph: row " + std::to_string(n) + " has an id after padding");
-                return SS_EINVAL;
-            }
-            if (id >= ix->num) {
-                set_error("ss_set_graph: row " + std::to_string(n) + " holds id " + std::to_string(id) +
-                          " >= num");
-                return SS_EINVAL;
-            }
-            bool dup = false;
-            for (uint32_t x = 0; x < w && n != ep; x++)
-                dup |= dst[x] == id;
-            if (!dup)
-                dst[w++] = id;
         }
+    };
+    std::vector<std::thread> workers;
+    for (uint32_t t = 1; t < nt; t++)
+        workers.emplace_back(check_rows, t);
+    check_rows(0);
+    for (auto &th : workers)
+        th.join();
+    for (uint32_t t = 0; t < nt; t++) {
+        if (bad_row[t] == SS_NONE)
+            continue;
+        if (bad_id[t] == SS_NONE)
+            set_error("ss_set_graph: row " + std::to_string(bad_row[t]) + " has an id after padding");
+        else
+            set_error("ss_set_graph: row " + std::to_string(bad_row[t]) + " holds id " + std::to_string(bad_id[t]) +
+                      " >= num");
+        return SS_EINVAL;
     }
     uint32_t *d = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), clean.size() * sizeof(uint32_t));
@@ -1176,6 +1199,158 @@ int ss_nn_descent(ss_index *ix, uint32_t k, uint32_t iters, uint32_t seed, uint3
         return hip_fail(e, "ss_nn_descent: download");
     for (size_t i = 0; i < nk; i++)
         out_ids[i] = h[i] == ss::kNone ? ss::kNone : (h[i] & ~ss::kExplored);
+    return SS_OK;
+}
+
+// ss_build_graph with its two prune steps on the GPU (prune_kernel): the
+// candidate lists are uploaded, every node's list is pruned on the device,
+// the reverse lists are built there (count, host prefix sum, fill) and every
+// node's list united with its reverse list is pruned again on the device;
+// nodes with more than kPruneMaxCand such candidates and the connectivity
+// pass run on the host.  The arithmetic and the orders are the host
+// builder's, so the graph is identical to ss_build_graph's.
+int ss_build_graph_gpu(ss_index *ix, const float *base, const uint32_t *knn, uint32_t knn_k, const uint32_t *extra,
+                       uint32_t extra_k, uint32_t R, float alpha, uint32_t ep, uint32_t *adj_out,
+                       uint32_t *unreachable_fixed)
+{
+    if (!ix || !base || !knn || !adj_out || R == 0 || R > SS_MAX_RPAD || ep >= ix->num || (extra_k && !extra) ||
+        !(alpha >= 1.0f) || knn_k == 0) {
+        set_error("ss_build_graph_gpu: bad argument");
+        return SS_EINVAL;
+    }
+    if ((size_t)knn_k + extra_k > ss::kPruneMaxCand) {
+        set_error("ss_build_graph_gpu: more than 512 candidates per node (knn_k + extra_k)");
+        return SS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ix->mu);
+    const size_t n = ix->num, nr = n * R;
+    hipStream_t st = ix->stream;
+    hipError_t e;
+    DevBuf<uint32_t> da, db, p_ids, p_deg, ctr;
+    DevBuf<float> p_d;
+    if ((e = da.reserve(n * knn_k)) || (extra_k && (e = db.reserve(n * extra_k))) || (e = p_ids.reserve(nr)) ||
+        (e = p_d.reserve(nr)) || (e = p_deg.reserve(n)) || (e = ctr.reserve(64)))
+        return hip_fail(e, "ss_build_graph_gpu: allocation");
+    if ((e = hipMemcpyAsync(da.ptr, knn, n * knn_k * 4, hipMemcpyHostToDevice, st)) ||
+        (extra_k && (e = hipMemcpyAsync(db.ptr, extra, n * extra_k * 4, hipMemcpyHostToDevice, st))))
+        return hip_fail(e, "ss_build_graph_gpu: upload of candidates");
+    ss::PruneArgs pa{};
+    pa.base = ix->d_base;
+    pa.num = ix->num;
+    pa.dim_pad = ix->dim_pad;
+    pa.R = R;
+    pa.alpha = alpha;
+    pa.queue = ctr.ptr;
+    const size_t lds = ss::prune_lds_bytes(ix->dim_pad);
+    const uint32_t waves_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(ix->lds_per_cu / lds, 16));
+    const uint32_t blocks = (uint32_t)ix->num_cus * waves_cu;
+    // 1. prune of every node's candidates
+    pa.mode = 0;
+    pa.a = da.ptr;
+    pa.ka = knn_k;
+    pa.b = extra_k ? db.ptr : nullptr;
+    pa.kb = extra_k;
+    pa.out_ids = p_ids.ptr;
+    pa.out_d = p_d.ptr;
+    pa.out_deg = p_deg.ptr;
+    if ((e = ss::launch_queue_reset(ctr.ptr, st)) || (e = ss::launch_prune(ix->metric, pa, blocks, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return hip_fail(e, "ss_build_graph_gpu: prune");
+    da.release();
+    db.release();
+    // 2. reverse lists: counts, offsets (host prefix sum), fill
+    DevBuf<uint32_t> cnt, fill, rev;
+    DevBuf<uint64_t> off;
+    if ((e = cnt.reserve(n)) || (e = fill.reserve(n)) || (e = off.reserve(n + 1)))
+        return hip_fail(e, "ss_build_graph_gpu: allocation");
+    ss::ReverseArgs ra{};
+    ra.ids = p_ids.ptr;
+    ra.deg = p_deg.ptr;
+    ra.num = ix->num;
+    ra.R = R;
+    ra.cnt = cnt.ptr;
+    ra.fill = fill.ptr;
+    std::vector<uint32_t> hcnt(n);
+    if ((e = ss::launch_clear_u32(cnt.ptr, n, st)) || (e = ss::launch_clear_u32(fill.ptr, n, st)) ||
+        (e = ss::launch_reverse(ra, 0, st)) ||
+        (e = hipMemcpyAsync(hcnt.data(), cnt.ptr, n * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return hip_fail(e, "ss_build_graph_gpu: reverse counts");
+    std::vector<uint64_t> hoff(n + 1);
+    hoff[0] = 0;
+    for (size_t u = 0; u < n; u++)
+        hoff[u + 1] = hoff[u] + hcnt[u];
+    if ((e = rev.reserve(std::max<uint64_t>(1, hoff[n]))) ||
+        (e = hipMemcpyAsync(off.ptr, hoff.data(), (n + 1) * 8, hipMemcpyHostToDevice, st)))
+        return hip_fail(e, "ss_build_graph_gpu: reverse offsets");
+    ra.off = off.ptr;
+    ra.rev_ids = rev.ptr;
+    if ((e = ss::launch_reverse(ra, 1, st)))
+        return hip_fail(e, "ss_build_graph_gpu: reverse fill");
+    // 3. each list united with its reverse list, pruned again if above R
+    DevBuf<uint32_t> f_ids, f_deg;
+    DevBuf<float> f_d;
+    if ((e = f_ids.reserve(nr)) || (e = f_d.reserve(nr)) || (e = f_deg.reserve(n)))
+        return hip_fail(e, "ss_build_graph_gpu: allocation");
+    pa.mode = 1;
+    pa.a = pa.b = nullptr;
+    pa.ka = pa.kb = 0;
+    pa.p_ids = p_ids.ptr;
+    pa.p_d = p_d.ptr;
+    pa.p_deg = p_deg.ptr;
+    pa.rev_off = off.ptr;
+    pa.rev_ids = rev.ptr;
+    pa.out_ids = f_ids.ptr;
+    pa.out_d = f_d.ptr;
+    pa.out_deg = f_deg.ptr;
+    std::vector<uint32_t> ids(nr), deg(n);
+    std::vector<float> ds(nr);
+    if ((e = ss::launch_queue_reset(ctr.ptr, st)) || (e = ss::launch_prune(ix->metric, pa, blocks, st)) ||
+        (e = hipMemcpyAsync(ids.data(), f_ids.ptr, nr * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(ds.data(), f_d.ptr, nr * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(deg.data(), f_deg.ptr, n * 4, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return hip_fail(e, "ss_build_graph_gpu: reverse prune");
+    // Nodes with too many candidates for the device's LDS lists (hubs named by
+    // thousands of pruned lists) are merged on the host, several at a time.
+    std::vector<uint32_t> over;
+    for (size_t u = 0; u < n; u++)
+        if (deg[u] == ss::kPruneOverflow)
+            over.push_back((uint32_t)u);
+    if (!over.empty()) {
+        std::vector<uint32_t> hp_ids(nr), hp_deg(n), hrev(hoff[n]);
+        std::vector<float> hp_d(nr);
+        if ((e = hipMemcpy(hp_ids.data(), p_ids.ptr, nr * 4, hipMemcpyDeviceToHost)) ||
+            (e = hipMemcpy(hp_d.data(), p_d.ptr, nr * 4, hipMemcpyDeviceToHost)) ||
+            (e = hipMemcpy(hp_deg.data(), p_deg.ptr, n * 4, hipMemcpyDeviceToHost)) ||
+            (!hrev.empty() && (e = hipMemcpy(hrev.data(), rev.ptr, hrev.size() * 4, hipMemcpyDeviceToHost))))
+            return hip_fail(e, "ss_build_graph_gpu: download of overflowing lists");
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < over.size();) {
+                const uint32_t u = over[i];
+                ss::merge_reverse_host(base, ix->dim, ix->metric, alpha, u, hp_ids.data() + (size_t)u * R,
+                                       hp_d.data() + (size_t)u * R, std::min<uint32_t>(hp_deg[u], R),
+                                       hrev.data() + hoff[u], hoff[u + 1] - hoff[u], R, ids.data() + (size_t)u * R,
+                                       ds.data() + (size_t)u * R, &deg[u]);
+            }
+        };
+        const uint32_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (uint32_t t = 1; t < nt; t++)
+            th.emplace_back(work);
+        work();
+        for (auto &t : th)
+            t.join();
+    }
+    uint32_t fixed = 0;
+    int rc = ss::connect_graph(base, ix->num, ix->dim, ix->metric, knn, knn_k, extra, extra_k, R, ep, ids.data(),
+                               ds.data(), deg.data(), &fixed);
+    if (rc)
+        return rc;
+    std::memcpy(adj_out, ids.data(), nr * 4);
+    if (unreachable_fixed)
+        *unreachable_fixed = fixed;
     return SS_OK;
 }
 

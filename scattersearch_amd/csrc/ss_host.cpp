@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/scattersearch.h"
+#include "ss_graph_host.h"
 
 namespace ss {
 void set_error(const std::string &msg);
@@ -150,6 +151,242 @@ void mrng_prune(const float *base, uint32_t dim, int metric, float alpha, std::v
 }
 
 }  // namespace
+
+namespace ss {
+
+float host_distance(int metric, const float *a, const float *b, uint32_t dim) { return host_dist(metric, a, b, dim); }
+
+void merge_reverse_host(const float *base, uint32_t dim, int metric, float alpha, uint32_t u, const uint32_t *ids,
+                        const float *dists, size_t n, const uint32_t *rev, size_t m, uint32_t R, uint32_t *out_ids,
+                        float *out_dists, uint32_t *out_deg)
+{
+    std::vector<Cand> merged(n);
+    for (size_t j = 0; j < n; j++)
+        merged[j] = Cand{dists[j], ids[j]};
+    const float *uv = base + (size_t)u * dim;
+    for (size_t j = 0; j < m; j++) {
+        const uint32_t v = rev[j];
+        bool have = false;
+        for (size_t x = 0; x < n && !have; x++)
+            have = ids[x] == v;
+        if (!have)
+            merged.push_back(Cand{host_dist(metric, uv, base + (size_t)v * dim, dim), v});
+    }
+    std::vector<Cand> fin;
+    if (merged.size() <= R) {
+        std::sort(merged.begin(), merged.end());
+        fin = std::move(merged);
+    } else {
+        mrng_prune(base, dim, metric, alpha, merged, R, fin);
+    }
+    for (uint32_t j = 0; j < R; j++) {
+        out_ids[j] = j < fin.size() ? fin[j].id : SS_NONE;
+        out_dists[j] = j < fin.size() ? fin[j].d : INFINITY;
+    }
+    *out_deg = (uint32_t)fin.size();
+}
+
+namespace {
+
+// Best-first search over the graph being built (rows ids/deg) from ep for
+// vector uv with a pool of pool_L entries; appends every node whose distance
+// it computed, with that distance, to `found`.  Every such node is reachable
+// from ep.  stamp/gen mark the visited nodes without clearing per call.
+void attach_search(const float *base, uint32_t dim, int metric, const uint32_t *ids, const uint32_t *deg,
+                   uint32_t R, uint32_t ep, const float *uv, uint32_t pool_L, std::vector<uint32_t> &stamp,
+                   uint32_t gen, std::vector<Cand> &found)
+{
+    struct Entry {
+        Cand c;
+        bool done;
+    };
+    std::vector<Entry> pool;
+    pool.reserve(pool_L + 1);
+    stamp[ep] = gen;
+    const Cand e0{host_dist(metric, base + (size_t)ep * dim, uv, dim), ep};
+    found.push_back(e0);
+    pool.push_back(Entry{e0, false});
+    for (size_t k = 0; k < pool.size();) {
+        if (pool[k].done) {
+            k++;
+            continue;
+        }
+        pool[k].done = true;
+        const uint32_t n = pool[k].c.id;
+        size_t next = pool.size();
+        for (uint32_t j = 0; j < deg[n]; j++) {
+            const uint32_t m = ids[(size_t)n * R + j];
+            if (stamp[m] == gen)
+                continue;
+            stamp[m] = gen;
+            const Cand c{host_dist(metric, base + (size_t)m * dim, uv, dim), m};
+            found.push_back(c);
+            if (pool.size() >= pool_L && !(c < pool.back().c))
+                continue;
+            auto it = std::upper_bound(pool.begin(), pool.end(), c,
+                                       [](const Cand &x, const Entry &y) { return x < y.c; });
+            const size_t pos = (size_t)(it - pool.begin());
+            pool.insert(it, Entry{c, false});
+            if (pool.size() > pool_L)
+                pool.pop_back();
+            next = std::min(next, pos);
+        }
+        k = next <= k ? next : k + 1;
+    }
+}
+
+}  // namespace
+
+int connect_graph(const float *base, uint32_t num, uint32_t dim, int metric, const uint32_t *knn, uint32_t knn_k,
+                  const uint32_t *extra, uint32_t extra_k, uint32_t R, uint32_t ep, uint32_t *ids, float *dists,
+                  uint32_t *deg, uint32_t *fixed_out)
+{
+    constexpr uint32_t kAttachL = 64;        // pool of the attach search
+    constexpr size_t kBatch = size_t(1) << 16;  // unreachable nodes whose targets are found together
+    std::vector<uint8_t> seen(num, 0);
+    std::vector<uint32_t> stack;
+    auto flood = [&](uint32_t s) {
+        seen[s] = 1;
+        stack.push_back(s);
+        while (!stack.empty()) {
+            const uint32_t n = stack.back();
+            stack.pop_back();
+            for (uint32_t j = 0; j < deg[n]; j++) {
+                const uint32_t c = ids[(size_t)n * R + j];
+                if (!seen[c]) {
+                    seen[c] = 1;
+                    stack.push_back(c);
+                }
+            }
+        }
+    };
+    flood(ep);
+    // Target of u on the graph as it stands (read only): its nearest reachable
+    // candidate with room, else the nearest reachable node with room that a
+    // search for u from ep reaches; SS_NONE if neither exists.
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<uint32_t>> stamps(nt);
+    std::vector<uint32_t> gens(nt, 0);
+    auto target = [&](uint32_t u, int t) -> Cand {
+        const float *uv = base + (size_t)u * dim;
+        Cand best{INFINITY, SS_NONE};
+        auto consider = [&](uint32_t w) {
+            if (w == SS_NONE || w >= num || !seen[w] || deg[w] >= R)
+                return;
+            Cand c{host_dist(metric, base + (size_t)w * dim, uv, dim), w};
+            if (c < best)
+                best = c;
+        };
+        for (uint32_t j = 0; j < knn_k; j++)
+            consider(knn[(size_t)u * knn_k + j]);
+        for (uint32_t j = 0; j < extra_k; j++)
+            consider(extra[(size_t)u * extra_k + j]);
+        if (best.id != SS_NONE)
+            return best;
+        if (stamps[t].empty())
+            stamps[t].assign(num, 0);
+        std::vector<Cand> found;
+        attach_search(base, dim, metric, ids, deg, R, ep, uv, kAttachL, stamps[t], ++gens[t], found);
+        for (const Cand &c : found)
+            if (deg[c.id] < R && c < best)
+                best = c;
+        return best;
+    };
+    std::vector<uint32_t> pending;
+    for (uint32_t u = 0; u < num; u++)
+        if (!seen[u])
+            pending.push_back(u);
+    uint32_t fixed = 0;
+    std::vector<Cand> tg;
+    std::vector<uint32_t> later;
+    size_t head = 0;
+    while (head < pending.size()) {
+        // targets of the next batch, found in parallel on the current graph
+        const size_t nb = std::min(kBatch, pending.size() - head);
+        tg.assign(nb, Cand{INFINITY, SS_NONE});
+        std::atomic<size_t> next{0};
+        auto work = [&](int t) {
+            for (size_t i; (i = next.fetch_add(64)) < nb;)
+                for (size_t k = i; k < std::min(nb, i + 64); k++)
+                    if (!seen[pending[head + k]])
+                        tg[k] = target(pending[head + k], t);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++)
+            th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th)
+            x.join();
+        // applied in id order: a node reached meanwhile needs nothing, one
+        // whose target filled up meanwhile is tried again in a later batch
+        // (the first node of a batch always attaches)
+        later.clear();
+        for (size_t k = 0; k < nb; k++) {
+            const uint32_t u = pending[head + k];
+            if (seen[u])
+                continue;
+            const float *uv = base + (size_t)u * dim;
+            Cand best = tg[k];
+            if (best.id != SS_NONE && deg[best.id] >= R) {
+                later.push_back(u);
+                continue;
+            }
+            if (best.id == SS_NONE) {
+                // no reachable node with room near u: scan every reachable node
+                std::vector<Cand> tb(nt, Cand{INFINITY, SS_NONE});
+                std::vector<std::thread> ts;
+                for (int t = 0; t < nt; t++) {
+                    ts.emplace_back([&, t]() {
+                        for (uint32_t w = t; w < num; w += nt) {
+                            if (!seen[w] || deg[w] >= R)
+                                continue;
+                            Cand c{host_dist(metric, base + (size_t)w * dim, uv, dim), w};
+                            if (c < tb[t])
+                                tb[t] = c;
+                        }
+                    });
+                }
+                for (auto &t : ts)
+                    t.join();
+                best = *std::min_element(tb.begin(), tb.end());
+            }
+            if (best.id == SS_NONE) {
+                set_error("ss_build_graph: every reachable node is full; raise R");
+                return SS_EINVAL;
+            }
+            const Cand e{host_dist(metric, base + (size_t)best.id * dim, uv, dim), u};
+            uint32_t *row = ids + (size_t)best.id * R;
+            float *rd = dists + (size_t)best.id * R;
+            uint32_t &n = deg[best.id];
+            uint32_t pos = n;  // after every entry not above e (upper bound)
+            while (pos > 0 && e < Cand{rd[pos - 1], row[pos - 1]})
+                pos--;
+            for (uint32_t j = n; j > pos; j--) {
+                row[j] = row[j - 1];
+                rd[j] = rd[j - 1];
+            }
+            row[pos] = e.id;
+            rd[pos] = e.d;
+            n++;
+            fixed++;
+            flood(u);
+        }
+        head += nb;
+        if (!later.empty()) {
+            // retried before the rest, in id order
+            std::vector<uint32_t> rest(pending.begin() + (ptrdiff_t)head, pending.end());
+            pending.swap(later);
+            pending.insert(pending.end(), rest.begin(), rest.end());
+            later.clear();
+            head = 0;
+        }
+    }
+    if (fixed_out)
+        *fixed_out = fixed;
+    return SS_OK;
+}
+
+}  // namespace ss
 
 // The "vecs" formats (load_fvecs, src/util.cpp:6-38): every record is
 // [i32 dim][dim elements]; fvecs elements are f32, ivecs i32, bvecs u8.  The
@@ -536,97 +773,29 @@ int ss_build_graph(const float *base, uint32_t num, uint32_t dim, int metric, co
     for (uint32_t v = 0; v < num; v++)
         for (const Cand &c : pruned[v])
             rev[c.id].push_back(v);
-    std::vector<std::vector<Cand>> fin(num);
+    std::vector<uint32_t> ids((size_t)num * R, SS_NONE), deg(num, 0);
+    std::vector<float> ds((size_t)num * R, INFINITY);
     parallel_for(num, nthreads, [&](uint32_t u) {
-        std::vector<Cand> merged = pruned[u];
-        const float *uv = base + (size_t)u * dim;
-        for (uint32_t v : rev[u]) {
-            bool have = false;
-            for (const Cand &c : merged)
-                have |= c.id == v;
-            if (!have)
-                merged.push_back(Cand{host_dist(metric, uv, base + (size_t)v * dim, dim), v});
+        std::vector<uint32_t> pid(pruned[u].size());
+        std::vector<float> pd(pruned[u].size());
+        for (size_t j = 0; j < pruned[u].size(); j++) {
+            pid[j] = pruned[u][j].id;
+            pd[j] = pruned[u][j].d;
         }
-        if (merged.size() <= R) {
-            std::sort(merged.begin(), merged.end());
-            fin[u] = std::move(merged);
-        } else {
-            mrng_prune(base, dim, metric, alpha, merged, R, fin[u]);
-        }
+        ss::merge_reverse_host(base, dim, metric, alpha, u, pid.data(), pd.data(), pid.size(), rev[u].data(),
+                               rev[u].size(), R, ids.data() + (size_t)u * R, ds.data() + (size_t)u * R, &deg[u]);
     });
     rev.clear();
     rev.shrink_to_fit();
-    // 3. Connectivity: attach every node not reachable from ep to its
-    //    nearest reachable candidate with a free slot.
-    std::vector<uint8_t> seen(num, 0);
-    std::vector<uint32_t> stack;
-    auto flood = [&](uint32_t s) {
-        seen[s] = 1;
-        stack.push_back(s);
-        while (!stack.empty()) {
-            uint32_t n = stack.back();
-            stack.pop_back();
-            for (const Cand &c : fin[n]) {
-                if (!seen[c.id]) {
-                    seen[c.id] = 1;
-                    stack.push_back(c.id);
-                }
-            }
-        }
-    };
-    flood(ep);
+    pruned.clear();
+    pruned.shrink_to_fit();
+    // 3. Connectivity.
     uint32_t fixed = 0;
-    for (uint32_t u = 0; u < num; u++) {
-        if (seen[u])
-            continue;
-        const float *uv = base + (size_t)u * dim;
-        Cand best{INFINITY, SS_NONE};
-        auto consider = [&](uint32_t w) {
-            if (w == SS_NONE || w >= num || !seen[w] || fin[w].size() >= R)
-                return;
-            Cand c{host_dist(metric, base + (size_t)w * dim, uv, dim), w};
-            if (c < best)
-                best = c;
-        };
-        for (uint32_t j = 0; j < knn_k; j++)
-            consider(knn[(size_t)u * knn_k + j]);
-        for (uint32_t j = 0; j < extra_k; j++)
-            consider(extra[(size_t)u * extra_k + j]);
-        if (best.id == SS_NONE) {
-            // No candidate qualifies: scan every reachable node with room.
-            const int nt = (int)std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
-            std::vector<Cand> tb(nt, Cand{INFINITY, SS_NONE});
-            std::vector<std::thread> ts;
-            for (int t = 0; t < nt; t++) {
-                ts.emplace_back([&, t]() {
-                    for (uint32_t w = t; w < num; w += nt) {
-                        if (!seen[w] || fin[w].size() >= R)
-                            continue;
-                        Cand c{host_dist(metric, base + (size_t)w * dim, uv, dim), w};
-                        if (c < tb[t])
-                            tb[t] = c;
-                    }
-                });
-            }
-            for (auto &t : ts)
-                t.join();
-            best = *std::min_element(tb.begin(), tb.end());
-        }
-        if (best.id == SS_NONE) {
-            set_error("ss_build_graph: every reachable node is full; raise R");
-            return SS_EINVAL;
-        }
-        Cand e{host_dist(metric, base + (size_t)best.id * dim, uv, dim), u};
-        auto &row = fin[best.id];
-        row.insert(std::upper_bound(row.begin(), row.end(), e), e);
-        fixed++;
-        flood(u);
-    }
-    for (uint32_t v = 0; v < num; v++) {
-        uint32_t *dst = adj_out + (size_t)v * R;
-        for (uint32_t j = 0; j < R; j++)
-            dst[j] = j < fin[v].size() ? fin[v][j].id : SS_NONE;
-    }
+    const int rc = ss::connect_graph(base, num, dim, metric, knn, knn_k, extra, extra_k, R, ep, ids.data(), ds.data(),
+                                     deg.data(), &fixed);
+    if (rc)
+        return rc;
+    std::memcpy(adj_out, ids.data(), ids.size() * sizeof(uint32_t));
     if (unreachable_fixed)
         *unreachable_fixed = fixed;
     return SS_OK;

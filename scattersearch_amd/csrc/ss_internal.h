@@ -109,6 +109,50 @@ struct NndArgs {
 };
 constexpr uint32_t kNndMaxKHost = 64;
 
+// Occlusion prune of candidate lists on the GPU (graph construction; see
+// ss_kernels.hip).  mode 0: the candidates of node v are lists a [num][ka]
+// and b [num][kb] (b may be null; SS_NONE, v itself and ids >= num are
+// skipped), distances computed here; the result is pruned by the occlusion
+// rule.  mode 1 (reverse edges): v's pruned list (p_ids/p_d/p_deg,
+// [num][R]) united with the nodes listing v (rev_ids[rev_off[v] ..
+// rev_off[v+1]]), pruned again only when it holds more than R entries.
+// Candidates are sorted by (distance, id); repeats are dropped.  Output rows
+// [num][R] of (id, distance) sorted by (distance, id), SS_NONE / +inf past
+// out_deg[v]; out_deg[v] = kPruneOverflow when v had more than
+// kPruneMaxCand candidates (mode 1 only: the host handles that node).
+constexpr uint32_t kPruneMaxCand = 512;
+constexpr uint32_t kPruneOverflow = 0xFFFFFFFFu;
+
+struct PruneArgs {
+    const float *base;
+    uint32_t num, dim_pad, R, mode;
+    float alpha;
+    const uint32_t *a, *b;
+    uint32_t ka, kb;
+    const uint32_t *p_ids;
+    const float *p_d;
+    const uint32_t *p_deg;
+    const uint64_t *rev_off;
+    const uint32_t *rev_ids;
+    uint32_t *out_ids;
+    float *out_d;
+    uint32_t *out_deg;
+    uint32_t *queue;
+};
+
+// Reverse lists of pruned rows: what 0 counts the entries naming each node
+// into cnt; what 1 places every entry v -> u at rev_ids[rev_off[u] + k]
+// (fill[] counts the places taken; the order inside a list is arbitrary).
+struct ReverseArgs {
+    const uint32_t *ids;   // [num][R]
+    const uint32_t *deg;   // [num]
+    uint32_t num, R;
+    uint32_t *cnt;         // [num]
+    const uint64_t *off;   // [num + 1]
+    uint32_t *fill;        // [num]
+    uint32_t *rev_ids;     // [off[num]]
+};
+
 struct BruteArgs {
     const float *base;
     const float *queries;
@@ -150,6 +194,9 @@ hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream);
 hipError_t launch_clear_u32(uint32_t *p, size_t n, hipStream_t stream);
 size_t nnd_lds_bytes_host(uint32_t dim_pad, uint32_t hash_slots);
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream);
+size_t prune_lds_bytes(uint32_t dim_pad);
+hipError_t launch_prune(int metric, const PruneArgs &a, uint32_t blocks, hipStream_t stream);
+hipError_t launch_reverse(const ReverseArgs &a, int what, hipStream_t stream);
 hipError_t launch_pad_copy(const float *src, float *dst, uint32_t rows, uint32_t dim,
                            uint32_t dim_pad, hipStream_t stream);
 

@@ -2260,6 +2260,266 @@ __global__ __launch_bounds__(64) void nnd_pull_kernel(NndArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Occlusion prune of candidate lists (graph construction; not on the query
+// path).  One wavefront per node v, the same steps as the host builder
+// (ss_host.cpp, mrng_prune and the reverse-edge merge) in the same
+// arithmetic, so the pruned rows are identical to the host's:
+//   1. gather v's candidates into LDS (skipping SS_NONE, v and ids >= num;
+//      an LDS set drops most repeats before their distances are computed);
+//   2. distances d(v, c) in the search's arithmetic, one candidate per lane;
+//   3. rank sort by (distance, id, position) and drop the repeats left
+//      (equal ids sit next to each other);
+//   4. the occlusion walk: c in sorted order is kept unless a kept s has
+//      alpha * d(s, c) < d(v, c), until R are kept; each step computes
+//      d(s, c) for every kept s at once, one per lane.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPruneHashSlots = 2048;
+constexpr uint32_t kPruneNeedD = 0x80000000u;  // candidate id flag: distance still to compute
+constexpr uint32_t kPruneChunks = kPruneMaxCand / 64;
+
+__host__ __device__ inline size_t prune_lds(uint32_t dim_pad)
+{
+    return align16(dim_pad * 4u) + kPruneMaxCand * 8u + kPruneHashSlots * 4u + 64u * 8u;
+}
+
+// Set insert that reports an id at most once where it is placed; an id not
+// placed within kMaxProbe probes is reported as new (its repeat is dropped
+// after the sort).
+__device__ __forceinline__ bool prune_first_sight(uint32_t *tab, uint32_t id)
+{
+    uint32_t s = (id * 0x9E3779B1u) >> (32u - 11u);  // kPruneHashSlots = 2^11
+    for (uint32_t p = 0; p < kMaxProbe; p++) {
+        const uint32_t old = atomicCAS(&tab[s], kNone, id);
+        if (old == kNone)
+            return true;
+        if (old == id)
+            return false;
+        s = (s + 1u) & (kPruneHashSlots - 1u);
+    }
+    return true;
+}
+
+template <int NBLK, int TAILI, int METRIC>
+__global__ __launch_bounds__(64) void prune_kernel(PruneArgs a)
+{
+    static_assert(kPruneHashSlots == 2048u, "prune_first_sight hashes to 11 bits");
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *p = smem;
+    float *qv = reinterpret_cast<float *>(p);
+    p += align16(a.dim_pad * 4u);
+    uint32_t *cid = reinterpret_cast<uint32_t *>(p);
+    p += kPruneMaxCand * 4u;
+    float *cd = reinterpret_cast<float *>(p);
+    p += kPruneMaxCand * 4u;
+    uint32_t *hash = reinterpret_cast<uint32_t *>(p);
+    p += kPruneHashSlots * 4u;
+    uint32_t *kid = reinterpret_cast<uint32_t *>(p);
+    float *kd = reinterpret_cast<float *>(p + 256);
+    const uint32_t lane = lane_id();
+    const int nblk_rt = (int)(a.dim_pad / 16);
+    const int tail_rt = (int)((a.dim_pad % 16) != 0);
+    while (true) {
+        uint32_t v = 0;
+        if (lane == 0)
+            v = atomicAdd(a.queue, 1u);
+        v = uniform(v);
+        if (v >= a.num)
+            break;
+        for (uint32_t i = lane; i < a.dim_pad; i += 64)
+            qv[i] = a.base[(size_t)v * a.dim_pad + i];
+        for (uint32_t i = lane; i < kPruneHashSlots; i += 64)
+            hash[i] = kNone;
+        wave_sync();
+        // 1. candidates
+        uint32_t n = 0;
+        bool overflow = false;
+        auto add_list = [&](const uint32_t *src, size_t count, bool known) {
+            for (size_t j0 = 0; j0 < count; j0 += 64) {
+                const size_t j = j0 + lane;
+                uint32_t id = j < count ? src[j] : kNone;
+                const bool valid = id != kNone && id != v && id < a.num;
+                const bool fresh = valid && prune_first_sight(hash, id);
+                const uint64_t m = ballot(fresh);
+                const uint32_t pos = n + lanes_below(m);
+                if (fresh && pos < kPruneMaxCand) {
+                    cid[pos] = known ? id : (id | kPruneNeedD);
+                    if (known)
+                        cd[pos] = a.p_d[(size_t)v * a.R + j];
+                }
+                n += (uint32_t)__popcll(m);
+            }
+        };
+        if (a.mode == 0) {
+            add_list(a.a + (size_t)v * a.ka, a.ka, false);
+            if (a.b)
+                add_list(a.b + (size_t)v * a.kb, a.kb, false);
+        } else {
+            const uint32_t deg = a.p_deg[v];
+            const uint64_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
+            if (deg > a.R || r1 < r0 || (uint64_t)deg + (r1 - r0) > kPruneMaxCand) {
+                overflow = true;
+            } else {
+                add_list(a.p_ids + (size_t)v * a.R, deg, true);
+                add_list(a.rev_ids + r0, (size_t)(r1 - r0), false);
+            }
+        }
+        wave_sync();
+        if (overflow || n > kPruneMaxCand) {  // the host prunes this node
+            if (lane == 0)
+                a.out_deg[v] = kPruneOverflow;
+            for (uint32_t j = lane; j < a.R; j += 64) {
+                a.out_ids[(size_t)v * a.R + j] = kNone;
+                a.out_d[(size_t)v * a.R + j] = __builtin_inff();
+            }
+            wave_sync();
+            continue;
+        }
+        // 2. distances d(v, c)
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t e = cid[i];
+            if (e & kPruneNeedD) {
+                const uint32_t id = e & ~kPruneNeedD;
+                cd[i] = lane_dist<NBLK, TAILI, METRIC>(a.base + (size_t)id * a.dim_pad, qv, nblk_rt, tail_rt);
+                cid[i] = id;
+            }
+        }
+        wave_sync();
+        // 3. rank sort by (distance key, id, position), then drop repeats
+        uint32_t rid[kPruneChunks], rank[kPruneChunks];
+        float rd[kPruneChunks];
+        uint64_t rkey[kPruneChunks];
+#pragma unroll
+        for (uint32_t r = 0; r < kPruneChunks; r++) {
+            const uint32_t i = r * 64u + lane;
+            rid[r] = i < n ? cid[i] : kNone;
+            rd[r] = i < n ? cd[i] : __builtin_inff();
+            rkey[r] = ((uint64_t)dist_key<METRIC>(rd[r]) << 32) | rid[r];
+            rank[r] = 0;
+        }
+        for (uint32_t j = 0; j < n; j++) {
+            const uint64_t kj = ((uint64_t)dist_key<METRIC>(cd[j]) << 32) | cid[j];
+#pragma unroll
+            for (uint32_t r = 0; r < kPruneChunks; r++) {
+                const uint32_t i = r * 64u + lane;
+                rank[r] += (kj < rkey[r] || (kj == rkey[r] && j < i)) ? 1u : 0u;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (uint32_t r = 0; r < kPruneChunks; r++) {
+            if (r * 64u + lane < n) {
+                cid[rank[r]] = rid[r];
+                cd[rank[r]] = rd[r];
+            }
+        }
+        wave_sync();
+        bool keep[kPruneChunks];
+#pragma unroll
+        for (uint32_t r = 0; r < kPruneChunks; r++) {
+            const uint32_t i = r * 64u + lane;
+            rid[r] = i < n ? cid[i] : kNone;
+            rd[r] = i < n ? cd[i] : __builtin_inff();
+            keep[r] = i < n && (i == 0 || cid[i - 1] != rid[r]);
+        }
+        wave_sync();
+        uint32_t nu = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kPruneChunks; r++) {
+            const uint64_t m = ballot(keep[r]);
+            if (keep[r]) {
+                cid[nu + lanes_below(m)] = rid[r];
+                cd[nu + lanes_below(m)] = rd[r];
+            }
+            nu += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        // 4. occlusion walk (mode 1 keeps a list of at most R as it is)
+        uint32_t kept = 0;
+        if (a.mode == 1 && nu <= a.R) {
+            kept = nu;
+            if (lane < nu) {
+                kid[lane] = cid[lane];
+                kd[lane] = cd[lane];
+            }
+        } else {
+            for (uint32_t i = 0; i < nu && kept < a.R; i++) {
+                const uint32_t c = cid[i];
+                const float dc = cd[i];
+                bool occl = false;
+                if (kept > 0) {
+                    float dd = __builtin_inff();
+                    if (lane < kept) {
+                        const uint32_t sid = kid[lane];
+                        dd = lane_dist<NBLK, TAILI, METRIC>(a.base + (size_t)c * a.dim_pad,
+                                                            a.base + (size_t)sid * a.dim_pad, nblk_rt, tail_rt);
+                    }
+                    occl = ballot(lane < kept && a.alpha * dd < dc) != 0;
+                }
+                if (!occl) {
+                    if (lane == 0) {
+                        kid[kept] = c;
+                        kd[kept] = dc;
+                    }
+                    kept++;
+                    wave_sync();
+                }
+            }
+        }
+        wave_sync();
+        for (uint32_t j = lane; j < a.R; j += 64) {
+            a.out_ids[(size_t)v * a.R + j] = j < kept ? kid[j] : kNone;
+            a.out_d[(size_t)v * a.R + j] = j < kept ? kd[j] : __builtin_inff();
+        }
+        if (lane == 0)
+            a.out_deg[v] = kept;
+        wave_sync();
+    }
+}
+
+__global__ void reverse_count_kernel(ReverseArgs a)
+{
+    const size_t total = (size_t)a.num * a.R;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)(i / a.R), j = (uint32_t)(i % a.R);
+        if (j < a.deg[v]) {
+            const uint32_t u = a.ids[i];
+            if (u < a.num)
+                atomicAdd(&a.cnt[u], 1u);
+        }
+    }
+}
+
+__global__ void reverse_fill_kernel(ReverseArgs a)
+{
+    const size_t total = (size_t)a.num * a.R;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)(i / a.R), j = (uint32_t)(i % a.R);
+        if (j < a.deg[v]) {
+            const uint32_t u = a.ids[i];
+            if (u < a.num) {
+                const uint32_t k = atomicAdd(&a.fill[u], 1u);
+                if (a.off[u] + k < a.off[u + 1])
+                    a.rev_ids[a.off[u] + k] = v;
+            }
+        }
+    }
+}
+
+template <int NBLK, int TAILI, int METRIC>
+struct PruneLaunch {
+    static hipError_t run(const PruneArgs &a, uint32_t blocks, hipStream_t st)
+    {
+        hipLaunchKernelGGL((prune_kernel<NBLK, TAILI, METRIC>), dim3(blocks), dim3(64), prune_lds(a.dim_pad), st, a);
+        return hipGetLastError();
+    }
+    static hipError_t attr(size_t max_lds)
+    {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(prune_kernel<NBLK, TAILI, METRIC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    }
+};
+
 template <int NBLK, int TAILI, int METRIC>
 struct NndLaunch {
     static hipError_t run(const NndArgs &a, int what, uint32_t blocks, hipStream_t st)
@@ -2682,6 +2942,26 @@ hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream
     return hipGetLastError();
 }
 
+size_t prune_lds_bytes(uint32_t dim_pad) { return prune_lds(dim_pad); }
+
+hipError_t launch_prune(int metric, const PruneArgs &a, uint32_t blocks, hipStream_t stream)
+{
+    if (a.R == 0 || a.R > 64 || (a.mode == 0 && (size_t)a.ka + (a.b ? a.kb : 0) > kPruneMaxCand))
+        return hipErrorInvalidValue;
+    if (metric == SS_METRIC_INNER_PRODUCT)
+        return dispatch_dim<PruneLaunch, SS_METRIC_INNER_PRODUCT>(a.dim_pad, a, blocks, stream);
+    return dispatch_dim<PruneLaunch, SS_METRIC_L2>(a.dim_pad, a, blocks, stream);
+}
+
+hipError_t launch_reverse(const ReverseArgs &a, int what, hipStream_t stream)
+{
+    if (what == 0)
+        hipLaunchKernelGGL(reverse_count_kernel, dim3(2048), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(reverse_fill_kernel, dim3(2048), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream)
 {
     if (metric == SS_METRIC_INNER_PRODUCT)
@@ -2729,6 +3009,10 @@ hipError_t init_kernel_attributes(size_t max_lds)
     if ((e = attr_all<ServeLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<BfLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<PruneLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
+        return e;
+    if ((e = attr_all<PruneLaunch, SS_METRIC_INNER_PRODUCT>(max_lds)) != hipSuccess)
         return e;
     return attr_all<BfLaunch, SS_METRIC_INNER_PRODUCT>(max_lds);
 }
