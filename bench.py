@@ -87,6 +87,8 @@ def main():
                     help="skip the wider-expansion and several-wavefronts-per-query measurements")
     ap.add_argument("--visited-slots", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--team-size", type=int, default=0, help="lanes per distance (4 or 8); 0 = tuned")
+    ap.add_argument("--pool-layout", type=int, default=0, help="SS_POOL_* (1 registers, 2 LDS); 0 = tuned")
     ap.add_argument("--query-order", choices=["auto", "index", "locality"], default="auto",
                     help="order in which the batched search takes the queries (results do not depend on it)")
     ap.add_argument("--graph-cache", default="",
@@ -171,11 +173,13 @@ def main():
             return "locality" if n >= ss._lib.SS_ORDER_MIN_BATCH and wq <= 1 else "index"
         return args.query_order
 
-    def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block, ew=0, wq=0):
+    def launch(L, use_graph=True, slots=args.visited_slots, wpb=args.waves_per_block, ew=0, wq=0,
+               team=args.team_size, layout=args.pool_layout):
         ix.search_device(dq.data_ptr(), nq, K, L, K, d_ids.data_ptr(), d_dist.data_ptr(),
                          d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=use_graph,
                          visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq,
-                         query_order=qorder)
+                         query_order=qorder, team_size=team if ew in (0, 1) and wq in (0, 1) else 0,
+                         pool_layout=layout if ew in (0, 1) and wq in (0, 1) else 0)
 
     def stats_host():
         st = d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
@@ -378,7 +382,8 @@ def main():
                    "metric": "L2" if w.metric == 0 else "inner_product", "recall@10": round(rec, 4),
                    "mean_ndc": round(float(ndc_exact.mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1),
                    "recomputed_ndc": round(float((st["ndc"] - ndc_exact).mean()), 1),
-                   "launch": dict({k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots")},
+                   "launch": dict({k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots",
+                                                         "team_size", "pool_layout")},
                                   query_order=order_used(nq))},
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
@@ -572,7 +577,8 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         "config": {"workload": w.name, "num": ix.info()["num"], "dim": w.dim, "nq": nq, "R": w.R,
                    "metric": "L2" if w.metric == 0 else "inner_product",
                    "K_mix": {str(k): int((Ks == k).sum()) for k in (1, 10, 100)}, "L_mean": float(Ls.mean()),
-                   "launch": {k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots")}},
+                   "launch": {k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots",
+                                                     "team_size", "pool_layout")}},
         "recall": rec,
         "mean_ndc": round(float(ndc_exact.mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1),
         "recomputed_ndc": round(float((st["ndc"] - ndc_exact).mean()), 1),

@@ -78,7 +78,27 @@ typedef struct ss_search_params {
                                  (2): queries near one another in space are
                                  taken together, so that their searches share
                                  cached vectors and rows */
+    uint32_t team_size;       /* lanes that compute one distance in the
+                                 one-wavefront search: 4 (8-byte loads, 16
+                                 vectors per pass) or 8 (4-byte loads, 8 per
+                                 pass, fewer registers: more wavefronts per
+                                 CU); the distances are the same bits for
+                                 both (the edge-parallel distance loop of
+                                 EdgeWiseMultiThreadSearch,
+                                 src/annsearch.cpp:1457-1584, on a wavefront);
+                                 0 = tuned, else 4 */
+    uint32_t pool_layout;     /* SS_POOL_AUTO (0): tuned, else
+                                 SS_POOL_REGISTERS for L <= 64;
+                                 SS_POOL_REGISTERS (1): pools of at most 64
+                                 entries are read into registers once per
+                                 step; SS_POOL_LDS (2): the pool is scanned
+                                 in LDS and read again only for steps that
+                                 admit a candidate (the only layout above 64) */
 } ss_search_params;
+
+#define SS_POOL_AUTO 0u
+#define SS_POOL_REGISTERS 1u
+#define SS_POOL_LDS 2u
 
 #define SS_ORDER_AUTO 0u
 #define SS_ORDER_INDEX 1u
@@ -228,12 +248,15 @@ int ss_knn_graph(ss_index *index, uint32_t k, uint32_t *out_ids, float *out_dist
 int ss_nn_descent(ss_index *index, uint32_t k, uint32_t iters, uint32_t seed, uint32_t *out_ids,
                   float *out_dists, uint32_t *out_updates);
 
-/* Autotuner: times launch configurations (expansion width, waves per block,
- * visited-hash size) for pool size L on a sample of queries and keeps the
- * fastest whose recall@K against exact brute force on the sample is within
- * max_recall_loss of the exact search (max_recall_loss = 0 keeps exact
- * results: expansion width 1).  The choice is stored in the index for L's
- * power-of-two bucket and used by later searches that leave those fields 0. */
+/* Autotuner: times launch configurations for pool size L on a sample of
+ * queries and keeps the fastest whose recall@K against exact brute force on
+ * the sample is within max_recall_loss of the exact search: first waves
+ * per block and visited-hash size (and, with a recall allowance, expansion
+ * width and wavefronts per query), then distance team size and pool layout
+ * at the best of those (these two never change results).  max_recall_loss
+ * = 0 keeps exact results.  The choice is stored in the index for L's
+ * power-of-two bucket (dimension and degree are the index's) and used by
+ * later searches that leave those fields 0. */
 int ss_autotune(ss_index *index, const float *queries, uint32_t nq, uint32_t K, uint32_t L,
                 float max_recall_loss, ss_search_params *best, float *best_ms,
                 float *best_recall);

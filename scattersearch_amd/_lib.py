@@ -35,6 +35,10 @@ SS_ORDER_LOCALITY = 2
 SS_ORDER_MIN_BATCH = 4096
 SS_NOALLOC_MAX_QUERIES = 131072
 
+SS_POOL_AUTO = 0
+SS_POOL_REGISTERS = 1
+SS_POOL_LDS = 2
+
 
 class SearchParams(ctypes.Structure):
     _fields_ = [
@@ -46,6 +50,8 @@ class SearchParams(ctypes.Structure):
         ("expand_width", ctypes.c_uint32),
         ("waves_per_query", ctypes.c_uint32),
         ("query_order", ctypes.c_uint32),
+        ("team_size", ctypes.c_uint32),
+        ("pool_layout", ctypes.c_uint32),
     ]
 
 

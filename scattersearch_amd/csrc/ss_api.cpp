@@ -126,7 +126,7 @@ struct ss_index {
     // Launch parameters chosen by ss_autotune, per pool-size bucket
     // (next power of two of L); dim and degree are fixed per index.
     struct Tuned {
-        uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query;
+        uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query, team_size, pool_layout;
     };
     std::vector<Tuned> tuned;
     // hipGraph cache of the last batched launch
@@ -493,6 +493,7 @@ struct Shape {
     size_t lds_wave;
     bool hash16;  // 16-bit visited slots (one wavefront per query)
     uint32_t hash_rem_bits, hash_id_mask;
+    uint32_t team, pool_lds;
 };
 
 
@@ -514,11 +515,17 @@ static ss_search_params resolve_params(const ss_index *ix, uint32_t l_max, const
             r.visited_slots = t.visited_slots;
         if (!r.waves_per_query)
             r.waves_per_query = t.waves_per_query;
+        if (!r.team_size)
+            r.team_size = t.team_size;
+        if (!r.pool_layout)
+            r.pool_layout = t.pool_layout;
     }
     if (!r.expand_width)
         r.expand_width = 1;
     if (!r.waves_per_query)
         r.waves_per_query = 1;
+    if (!r.team_size)
+        r.team_size = 4;
     return r;
 }
 
@@ -543,6 +550,22 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         set_error("waves_per_query must be 1..8, and expand_width 1 when it is above 1");
         return SS_EINVAL;
     }
+    const uint32_t team = p && p->team_size ? p->team_size : 4;
+    if (team != 4 && team != 8) {
+        set_error("team_size must be 4 or 8");
+        return SS_EINVAL;
+    }
+    if (team != 4 && (ew != 1 || wq != 1)) {
+        set_error("team_size 8 needs expand_width 1 and one wavefront per query");
+        return SS_EINVAL;
+    }
+    const uint32_t layout = p ? p->pool_layout : SS_POOL_AUTO;
+    if (layout > SS_POOL_LDS || (layout == SS_POOL_REGISTERS && l_max > 64)) {
+        set_error("pool_layout must be SS_POOL_AUTO, SS_POOL_REGISTERS (L <= 64) or SS_POOL_LDS");
+        return SS_EINVAL;
+    }
+    s->team = team;
+    s->pool_lds = layout == SS_POOL_LDS ? 1u : 0u;
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
     if (wq > 1) {  // blocks of wq wavefronts, one query per block
         const size_t per_block = ss::scatter_lds_bytes_per_block(ix->dim_pad, l_max, slots, wq);
@@ -562,8 +585,9 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_OK;
     }
     s->hash_rem_bits = s->hash_id_mask = 0;
-    s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu, &s->hash_rem_bits,
-                               &s->hash_id_mask);
+    s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu,
+                               ss::search_reg_waves_per_cu(ix->metric, ix->dim_pad, ew, l_max, team, s->pool_lds),
+                               &s->hash_rem_bits, &s->hash_id_mask);
     const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots, ew);
     const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
     if (waves_cu == 0) {
@@ -610,12 +634,15 @@ bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *i
 }
 
 bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
-                size_t lds_per_cu, uint32_t *rem_bits, uint32_t *id_mask)
+                size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t *rem_bits, uint32_t *id_mask)
 {
-    if (l_max <= 64 || expand_width != 1)  // the kernels for these keep the 32-bit table
+    if (expand_width != 1)  // the wider-expansion kernels keep the 32-bit table
         return false;
     const size_t lds_cu = lds_per_cu ? lds_per_cu : 65536;
-    if (lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots, expand_width) >= 16)
+    const size_t w32 = std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots, 1), reg_waves_cu);
+    const size_t w16 =
+        std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots / 2, 1), reg_waves_cu);
+    if (w16 <= w32)
         return false;
     return hash_params16(num, slots, rem_bits, id_mask);
 }
@@ -764,6 +791,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.expand_width = p && p->expand_width ? p->expand_width : 1;
     a.waves_per_query = p && p->waves_per_query ? p->waves_per_query : 1;
     a.num = ix->num;
+    a.team = sh.team;
+    a.pool_lds = sh.pool_lds;
     if (a.waves_per_query > 1 && d_path) {
         set_error("search: expansion paths are recorded by the one-wavefront search only");
         return SS_EINVAL;
@@ -1416,9 +1445,10 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
         float ms, rec;
     };
     std::vector<Cand> done;
-    // candidates: exact one-wavefront search over waves per block and visited
-    // slots; with a recall allowance also wider expansion and several
-    // wavefronts per query
+    // Stage 1 candidates: exact one-wavefront search over waves per block and
+    // visited slots (four-lane teams, the default pool layout); with a recall
+    // allowance also wider expansion and several wavefronts per query.
+    const uint32_t dflt_layout = L <= 64 ? SS_POOL_REGISTERS : SS_POOL_LDS;
     std::vector<ss_search_params> cands;
     const uint32_t wpbs[4] = {1, 2, 4, 8};
     const uint32_t slotss[4] = {1024, 2048, 4096, 8192};
@@ -1432,6 +1462,8 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                 p.waves_per_block = wpb;
                 p.visited_slots = slots;
                 p.waves_per_query = 1;
+                p.team_size = 4;
+                p.pool_layout = dflt_layout;
                 cands.push_back(p);
             }
     }
@@ -1443,15 +1475,19 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                 p.waves_per_block = wq;
                 p.visited_slots = slots;
                 p.waves_per_query = wq;
+                p.team_size = 4;
+                p.pool_layout = dflt_layout;
                 cands.push_back(p);
             }
     float exact_recall = -1.0f;
-    for (const ss_search_params &p : cands) {
+    // Times one candidate (a warm-up launch, then the mean of three) and
+    // records it with its recall; false when it does not fit.
+    auto measure = [&](const ss_search_params &p) -> bool {
         Shape sh;
         if (choose_shape(ix, L, nq, &p, &sh) != SS_OK)
-            continue;
+            return false;
         if ((rc = run_search(ix, dq.ptr, nq, nullptr, nullptr, K, L, K, &p, di.ptr, dd.ptr, nullptr, ix->stream)))
-            break;
+            return false;
         float total = 0.0f;
         for (int rep = 0; rep < 3 && !rc; rep++) {
             float ms = 0.0f;
@@ -1462,29 +1498,54 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                 (e = hipEventElapsedTime(&ms, ev0, ev1))) {
                 if (!rc)
                     rc = hip_fail(e, "ss_autotune: timing");
-                break;
+                return false;
             }
             total += ms;
         }
-        if (rc)
-            break;
         if ((e = hipMemcpy(got.data(), di.ptr, nk * 4, hipMemcpyDeviceToHost)) != hipSuccess) {
             rc = hip_fail(e, "ss_autotune: results");
-            break;
+            return false;
         }
         const float rec = recall(got);
-        if (p.expand_width == 1 && p.waves_per_query == 1)
+        if (p.expand_width == 1 && p.waves_per_query == 1 && p.team_size == 4 && exact_recall < 0.0f)
             exact_recall = rec;
         done.push_back(Cand{p, total / 3.0f, rec});
+        return true;
+    };
+    for (const ss_search_params &p : cands) {
+        measure(p);
+        if (rc)
+            break;
+    }
+    auto pick = [&]() {
+        const Cand *b = nullptr;
+        for (const Cand &c : done)
+            if (c.rec >= exact_recall - max_recall_loss && (!b || c.ms < b->ms))
+                b = &c;
+        return b;
+    };
+    // Stage 2: distance team size and pool layout at the best stage-1 shape of
+    // the one-wavefront exact search (neither changes results).
+    const Cand *b1 = rc ? nullptr : pick();
+    if (b1 && b1->p.expand_width == 1 && b1->p.waves_per_query == 1) {
+        const ss_search_params base_p = b1->p;
+        for (uint32_t team : {4u, 8u})
+            for (uint32_t layout : {SS_POOL_REGISTERS, SS_POOL_LDS}) {
+                if ((layout == SS_POOL_REGISTERS && L > 64) || (team == 4 && layout == dflt_layout))
+                    continue;
+                ss_search_params p = base_p;
+                p.team_size = team;
+                p.pool_layout = layout;
+                measure(p);
+                if (rc)
+                    break;
+            }
     }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     if (rc)
         return rc;
-    const Cand *best = nullptr;
-    for (const Cand &c : done)
-        if (c.rec >= exact_recall - max_recall_loss && (!best || c.ms < best->ms))
-            best = &c;
+    const Cand *best = pick();
     if (!best) {
         set_error("ss_autotune: no configuration fits this L");
         return SS_EINVAL;
@@ -1494,7 +1555,7 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                                    [&](const ss_index::Tuned &t) { return t.l_bucket == bucket; }),
                     ix->tuned.end());
     ix->tuned.push_back(ss_index::Tuned{bucket, best->p.expand_width, best->p.waves_per_block, best->p.visited_slots,
-                                        best->p.waves_per_query});
+                                        best->p.waves_per_query, best->p.team_size, best->p.pool_layout});
     if (best_out)
         *best_out = best->p;
     if (best_ms_out)

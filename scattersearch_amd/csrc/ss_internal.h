@@ -44,7 +44,19 @@ struct SearchArgs {
     uint32_t waves_per_query; // 1, or 2..8 wavefronts per query (ScatterSearch, blocks of that many)
     uint32_t num;          // base points (random entry points of the multi-wavefront search)
     const uint32_t *order; // non-null: the work queue's position i holds query order[i]
+    uint32_t team;         // lanes per distance (4 or 8) of the one-wavefront search
+    uint32_t pool_lds;     // 1: pools of at most 64 entries take the LDS-scanned body
 };
+
+// Pool chunks of 64 entries the search body keeps in registers for the
+// merge: 1 (the pool read once per step into registers, L <= 64) unless
+// pool_lds asks for the LDS-scanned body, which reads the pool again only
+// for steps that admit a candidate.
+constexpr uint32_t kPoolChunksMax = SS_MAX_L / 64;
+inline uint32_t pool_chunks(uint32_t l_max, uint32_t pool_lds)
+{
+    return (l_max <= 64 && !pool_lds) ? 1u : (l_max <= 128 ? 2u : (l_max <= 256 ? 4u : kPoolChunksMax));
+}
 
 // Locality order of a batched search (scheduling only).  Queries are keyed by
 // their nearest of kOrderPivots pivots (base points, first key_dims
@@ -252,14 +264,24 @@ int index_view(const ss_index *ix, IndexView *v);
 // whose top b bits pick the bucket and whose low n - b bits are stored, which
 // needs n - b <= 15 (bit 15 marks an empty slot).
 bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *id_mask);
-// Whether a one-wavefront-per-query launch uses 16-bit slots: for pools above
-// 64 entries expanded one node per step, when they fit (hash_params16) and
-// the 32-bit table leaves room for fewer than 16 wavefronts per CU (the most
-// the search kernels keep resident), so that the smaller table raises
-// residency; otherwise the 32-bit table, whose lookup and claim take fewer
-// instructions and LDS round trips.
+// Whether a one-wavefront-per-query launch (one node per step) uses 16-bit
+// slots: when they fit (hash_params16) and keep more wavefronts resident
+// per CU than the 32-bit table, given the LDS of a CU and reg_waves_cu, the
+// wavefronts per CU the kernel's registers allow; otherwise the 32-bit
+// table, whose lookup and claim take fewer instructions and LDS round trips.
 bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
-                size_t lds_per_cu, uint32_t *rem_bits, uint32_t *id_mask);
+                size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t *rem_bits, uint32_t *id_mask);
+// Wavefronts per CU the registers of the one-wavefront-per-query search
+// kernel for (metric, dim_pad, expand_width, pool bound, team, pool layout)
+// allow.
+uint32_t search_reg_waves_per_cu(int metric, uint32_t dim_pad, uint32_t expand_width, uint32_t l_max,
+                                 uint32_t team = 4, uint32_t pool_lds = 0);
+
+// Search kernels with teams of 8 lanes per distance (ss_search_team.hip, a
+// translation unit of its own): launch, LDS attribute set-up, VGPRs.
+hipError_t launch_search_team8(int metric, const SearchArgs &a, uint32_t blocks, uint32_t wpb, hipStream_t stream);
+hipError_t init_search_team8(size_t max_lds);
+int search_regs_team8(int metric, uint32_t dim_pad, uint32_t l_max, uint32_t pool_lds);
 // Launch shape (waves per block, visited slots, blocks) the batched search
 // would use for nq queries with pool bound l_max; 0 arguments pick defaults.
 int shape_for(const ss_index *ix, uint32_t l_max, uint32_t nq, uint32_t visited_slots, uint32_t waves_per_block,

@@ -316,7 +316,10 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     a.search.hash_shift = 32u - lg;
     uint32_t rem_bits = 0, id_mask = 0;
     a.search.hash16 =
-        ss::use_hash16(s->v.num, s->v.dim_pad, c.l_max, slots, 1, s->v.lds_per_cu, &rem_bits, &id_mask) ? 1u : 0u;
+        ss::use_hash16(s->v.num, s->v.dim_pad, c.l_max, slots, 1, s->v.lds_per_cu,
+                       ss::search_reg_waves_per_cu(s->v.metric, s->v.dim_pad, 1, c.l_max), &rem_bits, &id_mask)
+            ? 1u
+            : 0u;
     a.search.hash_rem_bits = rem_bits;
     a.search.hash_id_mask = id_mask;
     a.search.k_stride = c.k_max;

@@ -83,8 +83,11 @@ def test_status_codes_match_the_header():
         assert getattr(_lib, name) == defs[name], name
     assert ctypes.sizeof(_lib.QueryStats) == 32 == STATS_DTYPE.itemsize
     assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
-    assert ctypes.sizeof(_lib.SearchParams) == 32 and _lib.SearchParams.query_order.offset == 28
-    assert "uint32_t query_order;" in hdr
+    assert ctypes.sizeof(_lib.SearchParams) == 40 and _lib.SearchParams.query_order.offset == 28
+    assert _lib.SearchParams.team_size.offset == 32 and _lib.SearchParams.pool_layout.offset == 36
+    assert "uint32_t query_order;" in hdr and "uint32_t team_size;" in hdr and "uint32_t pool_layout;" in hdr
+    for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_NOALLOC_MAX_QUERIES"):
+        assert getattr(_lib, name) == defs[name], name
 
 
 def test_non_finite_base_is_refused():

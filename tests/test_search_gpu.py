@@ -80,6 +80,24 @@ def test_entry_row_repeats_kept_other_rows_deduplicated(small):
         _assert_same(gpu, ora, what=f"repeated entry-row id, L={L}")
 
 
+def test_small_pool_with_16bit_visited_slots_equals_oracle():
+    """200-d inner product at L = 48 with 8,192 visited slots: the 32-bit
+    table would allow 4 wavefronts per CU where the registers allow 8, so
+    the one-read register pool runs with the 16-bit table; results equal the
+    oracle (ids, distance bits, NDC, hops)."""
+    import oracle
+    from scattersearch_amd import data
+    from scattersearch_amd.graph import medoid
+    base, q = data.mixture(4000, 100, 200, 16, 0.25, seed=61, lognormal_sigma=0.5, unit_norm=True)
+    knn = oracle.bruteforce(base, base, 24, 1, exclude=np.arange(4000, dtype=np.uint32))[0]
+    ep = medoid(base)
+    ix = _index(base, knn, ep, metric=1)
+    for K, L in ((10, 48), (1, 20), (48, 64)):
+        gpu = ix.search(q, K, L, visited_slots=8192)
+        ora = oracle.search_batch(base, knn, ep, q, K, L, 1)
+        _assert_same(gpu, ora, what=f"16-bit visited slots, K={K} L={L}")
+
+
 def test_per_query_launch_equals_batched(small):
     ix = _index(small["base"], small["adj"], small["ep"])
     from scattersearch_amd import SS_MODE_PER_QUERY
