@@ -73,8 +73,12 @@ typedef struct ss_search_params {
                                  queries (results do not depend on it):
                                  SS_ORDER_AUTO (0): SS_ORDER_LOCALITY for
                                  batches of at least SS_ORDER_MIN_BATCH queries
-                                 with one wavefront per query, else index
-                                 order; SS_ORDER_INDEX (1); SS_ORDER_LOCALITY
+                                 with one wavefront per query and one pool
+                                 class (see pool_classes), longest pool first
+                                 within each class for a launch split into
+                                 several, else index order; SS_ORDER_INDEX
+                                 (1): no reordering beyond the grouping by
+                                 class; SS_ORDER_LOCALITY
                                  (2): queries near one another in space are
                                  taken together, so that their searches share
                                  cached vectors and rows */
@@ -94,7 +98,22 @@ typedef struct ss_search_params {
                                  step; SS_POOL_LDS (2): the pool is scanned
                                  in LDS and read again only for steps that
                                  admit a candidate (the only layout above 64) */
+    uint32_t pool_classes;    /* launches with per-query L above 64
+                                 (scheduling only, results do not depend on
+                                 it): SS_CLASSES_AUTO (0): the queries are
+                                 grouped by pool size (L <= 64, <= 128,
+                                 <= 256, <= 512) and each group is searched by
+                                 a kernel whose LDS is carved for its own
+                                 largest pool, the groups' kernels running
+                                 side by side; SS_CLASSES_SERIAL (2): the same
+                                 kernels one after another; SS_CLASSES_OFF
+                                 (1): one kernel sized for the launch's
+                                 largest L */
 } ss_search_params;
+
+#define SS_CLASSES_AUTO 0u
+#define SS_CLASSES_OFF 1u
+#define SS_CLASSES_SERIAL 2u
 
 #define SS_POOL_AUTO 0u
 #define SS_POOL_REGISTERS 1u

@@ -39,6 +39,10 @@ SS_POOL_AUTO = 0
 SS_POOL_REGISTERS = 1
 SS_POOL_LDS = 2
 
+SS_CLASSES_AUTO = 0
+SS_CLASSES_OFF = 1
+SS_CLASSES_SERIAL = 2
+
 
 class SearchParams(ctypes.Structure):
     _fields_ = [
@@ -52,6 +56,7 @@ class SearchParams(ctypes.Structure):
         ("query_order", ctypes.c_uint32),
         ("team_size", ctypes.c_uint32),
         ("pool_layout", ctypes.c_uint32),
+        ("pool_classes", ctypes.c_uint32),
     ]
 
 

@@ -129,11 +129,21 @@ struct ss_index {
         uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query, team_size, pool_layout;
     };
     std::vector<Tuned> tuned;
-    // hipGraph cache of the last batched launch
+    // Streams and events that let the kernels of a launch's pool-size
+    // classes run side by side (fork from the caller's stream, join back).
+    static constexpr int kAux = (int)ss::kMaxPoolClasses - 1;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kAux] = {};
+    // hipGraph cache of the last batched launch: its kernels' arguments and shapes
+    struct Part {
+        ss::SearchArgs a;
+        uint32_t blocks, wpb;
+    };
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    ss::SearchArgs graph_args{};
-    uint32_t graph_blocks = 0, graph_wpb = 0;
+    std::vector<Part> graph_parts;
+    uint32_t graph_serial = 0;
     hipStream_t graph_stream = nullptr;
 };
 
@@ -170,6 +180,13 @@ static int init_device(ss_index *ix)
     // the current device is ix->device here
     if ((e = ss::scratch_address(ix->device, &ix->d_scratch)) != hipSuccess)
         return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if ((e = hipEventCreateWithFlags(&ix->fork_ev, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    for (int i = 0; i < ss_index::kAux; i++) {
+        if ((e = hipStreamCreateWithFlags(&ix->aux[i], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&ix->join_ev[i], hipEventDisableTiming)) != hipSuccess)
+            return hip_fail(e, "hipStreamCreate");
+    }
     return SS_OK;
 }
 
@@ -303,6 +320,16 @@ int ss_destroy(ss_index *ix)
     ix->dists.release();
     ix->stats.release();
     ix->path.release();
+    for (int i = 0; i < ss_index::kAux; i++) {
+        if (ix->aux[i]) {
+            (void)hipStreamSynchronize(ix->aux[i]);
+            (void)hipStreamDestroy(ix->aux[i]);
+        }
+        if (ix->join_ev[i])
+            (void)hipEventDestroy(ix->join_ev[i]);
+    }
+    if (ix->fork_ev)
+        (void)hipEventDestroy(ix->fork_ev);
     if (ix->stream)
         (void)hipStreamDestroy(ix->stream);
     delete ix;
@@ -756,6 +783,11 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         set_error("search: query_order must be SS_ORDER_AUTO, SS_ORDER_INDEX or SS_ORDER_LOCALITY");
         return SS_EINVAL;
     }
+    if (p && p->pool_classes > SS_CLASSES_SERIAL) {
+        set_error("search: pool_classes must be SS_CLASSES_AUTO, SS_CLASSES_OFF or SS_CLASSES_SERIAL");
+        return SS_EINVAL;
+    }
+    const ss_search_params *p_user = p;
     const ss_search_params resolved = resolve_params(ix, l_max, p);
     p = &resolved;
     Shape sh;
@@ -809,43 +841,132 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         return SS_OK;
     }
     a.queue = scratch ? scratch : ix->d_queue;
+    // Pool-size classes: a launch with per-query L above 64 searches each
+    // class of L with a kernel whose LDS is carved for that class's largest
+    // pool (more wavefronts per CU for the smaller ones); the queue words
+    // 0..3 count the classes' positions and words 8..15 hold their ranges.
+    uint32_t classes = 1;
+    const uint32_t cls_mode = p_user ? p_user->pool_classes : SS_CLASSES_AUTO;
+    if (d_L && !d_path && a.waves_per_query == 1 && a.expand_width == 1 && l_max > 64 && cls_mode != SS_CLASSES_OFF)
+        while (classes < ss::kMaxPoolClasses && (64u << (classes - 1)) < l_max)
+            classes++;
+    a.classes = classes;
+    // A launch split by pool size takes each class longest pool first unless
+    // the locality order is asked for by name.
     const bool locality = !d_path && a.waves_per_query == 1 &&
                           (p->query_order == SS_ORDER_LOCALITY ||
-                           (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH));
+                           (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH && classes == 1));
     ss::OrderArgs oa;
     std::memset(&oa, 0, sizeof(oa));
-    if (locality) {
+    if (locality || classes > 1) {
         oa.queries = d_queries;
         oa.nq = nq;
         oa.dim_pad = ix->dim_pad;
         oa.queue = a.queue;
+        oa.queues = a.queue;
+        oa.classes = classes;
+        oa.Ls = classes > 1 ? d_L : nullptr;
+        oa.ranges = classes > 1 ? a.queue + 8 : nullptr;
+        oa.longest_first = classes > 1 && p->query_order == SS_ORDER_AUTO ? 1u : 0u;
         if (scratch) {
             // pivots of this index, made again each call: every index on the
             // device shares the array
-            oa.key_dims = std::min<uint32_t>(ix->dim_pad, ss::kOrderKeyDims);
-            float *piv = reinterpret_cast<float *>(scratch + ss::kScratchPivotOff);
-            if ((e = ss::launch_order_pivots(ix->d_base, ix->dim_pad, ix->num, oa.key_dims, piv, st)) != hipSuccess)
-                return hip_fail(e, "order pivots");
-            oa.pivots = piv;
             oa.keys = scratch + ss::kScratchKeysOff;
             oa.order = scratch + ss::kScratchOrderOff;
+            if (locality) {
+                oa.key_dims = std::min<uint32_t>(ix->dim_pad, ss::kOrderKeyDims);
+                float *piv = reinterpret_cast<float *>(scratch + ss::kScratchPivotOff);
+                if ((e = ss::launch_order_pivots(ix->d_base, ix->dim_pad, ix->num, oa.key_dims, piv, st)) !=
+                    hipSuccess)
+                    return hip_fail(e, "order pivots");
+                oa.pivots = piv;
+            }
         } else {
             if ((rc = prepare_order(ix, nq, st)) != SS_OK)
                 return rc;
-            oa.pivots = ix->d_pivots;
-            oa.key_dims = ix->pivot_dims;
             oa.keys = ix->order.ptr;
             oa.order = ix->order.ptr + ix->order_n;
+            if (locality) {
+                oa.pivots = ix->d_pivots;
+                oa.key_dims = ix->pivot_dims;
+            }
         }
         a.order = oa.order;
     }
-    auto reset = [&]() {
-        return locality ? ss::launch_order(oa, (uint32_t)ix->num_cus, st) : ss::launch_queue_reset(a.queue, st);
+    // The kernels of the launch, the largest pools first (a fixed array: the
+    // no-allocation entry point takes this path too).
+    ss_index::Part parts_buf[ss::kMaxPoolClasses];
+    size_t nparts = 0;
+    if (classes == 1) {
+        parts_buf[nparts++] = ss_index::Part{a, sh.blocks, sh.wpb};
+    } else {
+        for (int c = (int)classes - 1; c >= 0; c--) {
+            const uint32_t bound = std::min<uint32_t>(64u << c, l_max);
+            ss_search_params pc = resolve_params(ix, bound, p_user);
+            pc.expand_width = 1;  // the class kernels run the one-wavefront search
+            pc.waves_per_query = 1;
+            Shape shc;
+            if ((rc = choose_shape(ix, bound, nq, &pc, &shc)))
+                return rc;
+            ss::SearchArgs ac = a;
+            ac.l_max = bound;
+            ac.hash_slots = shc.slots;
+            ac.hash_shift = 32u - log2u(shc.slots);
+            ac.hash16 = shc.hash16 ? 1u : 0u;
+            ac.hash_rem_bits = shc.hash_rem_bits;
+            ac.hash_id_mask = shc.hash_id_mask;
+            ac.team = shc.team;
+            ac.pool_lds = shc.pool_lds;
+            ac.queue = a.queue + c;
+            ac.range = a.queue + 8 + 2 * c;
+            parts_buf[nparts++] = ss_index::Part{ac, shc.blocks, shc.wpb};
+        }
+    }
+    struct PartList {
+        const ss_index::Part *p;
+        size_t n;
+        size_t size() const { return n; }
+        const ss_index::Part &operator[](size_t i) const { return p[i]; }
+        const ss_index::Part *begin() const { return p; }
+        const ss_index::Part *end() const { return p + n; }
+    } parts{parts_buf, nparts};
+    const bool serial = cls_mode == SS_CLASSES_SERIAL || parts.size() == 1;
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t x = (locality || classes > 1) ? ss::launch_order(oa, (uint32_t)ix->num_cus, st)
+                                                 : ss::launch_queue_reset(a.queue, st);
+        if (x != hipSuccess)
+            return x;
+        if (serial) {
+            for (const auto &pt : parts)
+                if ((x = ss::launch_search(ix->metric, pt.a, pt.blocks, pt.wpb, st)) != hipSuccess)
+                    return x;
+            return hipSuccess;
+        }
+        // fork: part 0 (the largest pools) on the caller's stream, the others
+        // on the index's auxiliary streams; join back before returning
+        if ((x = hipEventRecord(ix->fork_ev, st)) != hipSuccess)
+            return x;
+        for (size_t i = 1; i < parts.size(); i++)
+            if ((x = hipStreamWaitEvent(ix->aux[i - 1], ix->fork_ev, 0)) != hipSuccess)
+                return x;
+        if ((x = ss::launch_search(ix->metric, parts[0].a, parts[0].blocks, parts[0].wpb, st)) != hipSuccess)
+            return x;
+        for (size_t i = 1; i < parts.size(); i++) {
+            if ((x = ss::launch_search(ix->metric, parts[i].a, parts[i].blocks, parts[i].wpb, ix->aux[i - 1])) !=
+                    hipSuccess ||
+                (x = hipEventRecord(ix->join_ev[i - 1], ix->aux[i - 1])) != hipSuccess ||
+                (x = hipStreamWaitEvent(st, ix->join_ev[i - 1], 0)) != hipSuccess)
+                return x;
+        }
+        return hipSuccess;
     };
     const bool use_graph = p && p->use_graph && !d_path && !scratch;
     if (use_graph) {
-        const bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_blocks == sh.blocks &&
-                          ix->graph_wpb == sh.wpb && std::memcmp(&ix->graph_args, &a, sizeof(a)) == 0;
+        bool same = ix->graph_exec && ix->graph_stream == st && ix->graph_serial == (serial ? 1u : 0u) &&
+                    ix->graph_parts.size() == parts.size();
+        for (size_t i = 0; same && i < parts.size(); i++)
+            same = ix->graph_parts[i].blocks == parts[i].blocks && ix->graph_parts[i].wpb == parts[i].wpb &&
+                   std::memcmp(&ix->graph_parts[i].a, &parts[i].a, sizeof(ss::SearchArgs)) == 0;
         if (!same) {
             if (ix->graph_exec) {
                 (void)hipGraphExecDestroy(ix->graph_exec);
@@ -857,14 +978,13 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             }
             if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return hip_fail(e, "hipStreamBeginCapture");
-            hipError_t e1 = reset();
-            hipError_t e2 = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st);
+            hipError_t e1 = enqueue();
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(st, &g);
-            if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess) {
+            if (e1 != hipSuccess || e != hipSuccess) {
                 if (g)
                     (void)hipGraphDestroy(g);
-                return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e), "search graph capture");
+                return hip_fail(e1 != hipSuccess ? e1 : e, "search graph capture");
             }
             hipGraphExec_t ge = nullptr;
             if ((e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) != hipSuccess) {
@@ -873,18 +993,15 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             }
             ix->graph = g;
             ix->graph_exec = ge;
-            ix->graph_args = a;
-            ix->graph_blocks = sh.blocks;
-            ix->graph_wpb = sh.wpb;
+            ix->graph_parts.assign(parts.begin(), parts.end());
+            ix->graph_serial = serial ? 1u : 0u;
             ix->graph_stream = st;
         }
         if ((e = hipGraphLaunch(ix->graph_exec, st)) != hipSuccess)
             return hip_fail(e, "hipGraphLaunch");
         return SS_OK;
     }
-    if ((e = reset()) != hipSuccess)
-        return hip_fail(e, "queue reset");
-    if ((e = ss::launch_search(ix->metric, a, sh.blocks, sh.wpb, st)) != hipSuccess)
+    if ((e = enqueue()) != hipSuccess)
         return hip_fail(e, "search launch");
     return SS_OK;
 }

@@ -46,6 +46,8 @@ struct SearchArgs {
     const uint32_t *order; // non-null: the work queue's position i holds query order[i]
     uint32_t team;         // lanes per distance (4 or 8) of the one-wavefront search
     uint32_t pool_lds;     // 1: pools of at most 64 entries take the LDS-scanned body
+    const uint32_t *range; // non-null: this launch takes work-queue positions range[0] .. +range[1]
+    uint32_t classes;      // pool-size classes of the launch this kernel is part of (cache key)
 };
 
 // Pool chunks of 64 entries the search body keeps in registers for the
@@ -66,13 +68,25 @@ inline uint32_t pool_chunks(uint32_t l_max, uint32_t pool_lds)
 constexpr uint32_t kOrderPivots = 64;
 constexpr uint32_t kOrderKeyDims = 32;
 
+// Pool-size classes of a launch with per-query L: class b holds the queries
+// with L in (64 * 2^(b-1), 64 * 2^b] (class 0: L <= 64), searched by a kernel
+// whose LDS is carved for that class's largest pool.
+constexpr uint32_t kMaxPoolClasses = 4;
+
 struct OrderArgs {
     const float *queries;  // [nq][dim_pad]
-    const float *pivots;   // [key_dims][kOrderPivots]
-    uint32_t nq, dim_pad, key_dims;
+    const float *pivots;   // [key_dims][kOrderPivots]; unused when key_dims == 0
+    uint32_t nq, dim_pad;
+    uint32_t key_dims;     // 0: no locality key (the order only groups pool-size classes)
     uint32_t *keys;        // [nq] scratch
     uint32_t *order;       // [nq] out: query of each sorted position
-    uint32_t *queue;       // work-queue counter, cleared here
+    uint32_t *queue;       // work-queue counter of class 0, cleared here
+    // pool-size classes (classes > 1 only with Ls): key = class * kOrderPivots + pivot
+    const uint32_t *Ls;    // [nq] per-query L, or null
+    uint32_t classes;      // 1..kMaxPoolClasses
+    uint32_t *queues;      // [classes] counters cleared here (queues[0] == queue)
+    uint32_t *ranges;      // [2 * classes] out: first sorted position and count of each class
+    uint32_t longest_first;  // key_dims == 0: order each class by L, longest first
 };
 
 // A serving activation: wavefronts claim published requests from a ring in

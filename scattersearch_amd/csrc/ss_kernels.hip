@@ -49,19 +49,53 @@ __global__ __launch_bounds__(256) void order_pivot_kernel(const float *base, uin
 // transposed pivots) and a wavefront keys one query after another, the next
 // query's coordinates (lane i holds coordinate i) read while the current one
 // is keyed.  Key = the nearest pivot in squared distance (lowest index on
-// ties).  Block 0 also clears the work-queue counter.
+// ties), plus kOrderPivots times the query's pool-size class when the launch
+// has several; with no pivots (key_dims 0) and longest_first set, the place
+// of the query's L in its class, longest first, takes the pivot's place, so that each class kernel
+// starts its longest searches first and ends on short ones.  Block 0 also
+// clears the classes' work-queue counters.
 constexpr uint32_t kOrderKeyWaves = 4;
+
+__device__ __forceinline__ uint32_t pool_class(uint32_t L, uint32_t classes)
+{
+    uint32_t c = 0;
+    while (c + 1 < classes && L > (64u << c))
+        c++;
+    return c;
+}
+
+// Place of L among the pool sizes of its class, longest first: 0 for the
+// largest L the class holds (and for L beyond it, which the search rejects
+// at once), kOrderPivots - 1 for the smallest.
+__device__ __forceinline__ uint32_t longest_first(uint32_t L, uint32_t c)
+{
+    const uint32_t hi = 64u << c, lo = c ? (32u << c) + 1u : 1u;
+    if (L >= hi)
+        return 0u;
+    if (L < lo)
+        return kOrderPivots - 1u;
+    return (uint32_t)(((uint64_t)(hi - L) * kOrderPivots) / (hi - lo + 1u));
+}
 
 __global__ __launch_bounds__(64 * kOrderKeyWaves) void order_key_kernel(OrderArgs a)
 {
     static_assert(kOrderKeyDims <= 64, "one query coordinate per lane");
     const uint32_t lane = threadIdx.x & 63u;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        a.queue[0] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < a.classes)
+        a.queues[threadIdx.x] = 0u;
     const uint32_t stride = gridDim.x * kOrderKeyWaves;
     uint32_t q = blockIdx.x * kOrderKeyWaves + (threadIdx.x >> 6);
     if (q >= a.nq)
         return;
+    if (a.key_dims == 0) {  // classes only: each class longest pool first
+        for (; q < a.nq; q += stride)
+            if (lane == 0) {
+                const uint32_t L = a.Ls ? a.Ls[q] : 0u;
+                const uint32_t c = a.Ls ? pool_class(L, a.classes) : 0u;
+                a.keys[q] = c * kOrderPivots + (a.Ls && a.longest_first ? longest_first(L, c) : 0u);
+            }
+        return;
+    }
     float pv[kOrderKeyDims];
 #pragma unroll
     for (uint32_t i = 0; i < kOrderKeyDims; i++)
@@ -87,26 +121,29 @@ __global__ __launch_bounds__(64 * kOrderKeyWaves) void order_key_kernel(OrderArg
         const uint64_t hit = ballot(acc == m);  // empty only if every distance is NaN
         const uint32_t key = hit ? (uint32_t)__ffsll((unsigned long long)hit) - 1u : 0u;
         if (lane == 0)
-            a.keys[q] = key;
+            a.keys[q] = key + (a.Ls ? pool_class(a.Ls[q], a.classes) : 0u) * kOrderPivots;
         qc = qnext;
     }
 }
 
 // Counting sort of the keys by one workgroup: histogram, exclusive scan (one
-// wavefront), scatter (positions within one key in any order).  Batches of
-// up to kOrderSortThreads * kOrderSortUnroll queries keep their keys in
-// registers between the two passes.
+// wavefront, four keys per lane), scatter (positions within one key in any
+// order), and each class's range of sorted positions.  Batches of up to
+// kOrderSortThreads * kOrderSortUnroll queries keep their keys in registers
+// between the two passes.
 constexpr uint32_t kOrderSortThreads = 1024;
 constexpr uint32_t kOrderSortUnroll = 16;
+constexpr uint32_t kOrderKeys = kOrderPivots * kMaxPoolClasses;
 
 __global__ __launch_bounds__(kOrderSortThreads) void order_sort_kernel(OrderArgs a)
 {
-    __shared__ uint32_t cnt[kOrderPivots];
-    __shared__ uint32_t off[kOrderPivots];
+    __shared__ uint32_t cnt[kOrderKeys];
+    __shared__ uint32_t off[kOrderKeys];
     const uint32_t t = threadIdx.x;
+    const uint32_t nkeys = kOrderPivots * a.classes;
     constexpr uint32_t kStep = kOrderSortThreads * kOrderSortUnroll;
     const bool one_pass = a.nq <= kStep;
-    if (t < kOrderPivots)
+    if (t < kOrderKeys)
         cnt[t] = 0u;
     __syncthreads();
     uint32_t k[kOrderSortUnroll];
@@ -114,7 +151,7 @@ __global__ __launch_bounds__(kOrderSortThreads) void order_sort_kernel(OrderArgs
 #pragma unroll
         for (uint32_t u = 0; u < kOrderSortUnroll; u++) {
             const uint32_t q = q0 + u * kOrderSortThreads + t;
-            k[u] = q < a.nq ? min(a.keys[q], kOrderPivots - 1u) : kNone;
+            k[u] = q < a.nq ? min(a.keys[q], nkeys - 1u) : kNone;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kOrderSortUnroll; u++)
@@ -122,17 +159,34 @@ __global__ __launch_bounds__(kOrderSortThreads) void order_sort_kernel(OrderArgs
                 atomicAdd(&cnt[k[u]], 1u);
     }
     __syncthreads();
-    static_assert(kOrderPivots == 64, "the scan gives one lane of a wavefront to each key");
+    static_assert(kOrderKeys == 4 * 64, "the scan gives four keys to each lane of a wavefront");
     if (t < 64) {
-        const uint32_t c = cnt[t];
-        uint32_t incl = c;
+        uint32_t c[4], local = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            c[i] = cnt[4 * t + i];
+            local += c[i];
+        }
+        uint32_t incl = local;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
             if ((int)t >= d)
                 incl += y;
         }
-        off[t] = incl - c;
+        uint32_t run = incl - local;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            off[4 * t + i] = run;
+            run += c[i];
+        }
+    }
+    __syncthreads();
+    if (t < a.classes && a.ranges) {
+        const uint32_t b0 = off[t * kOrderPivots];
+        const uint32_t b1 = t + 1 < a.classes ? off[(t + 1) * kOrderPivots] : a.nq;
+        a.ranges[2 * t] = b0;
+        a.ranges[2 * t + 1] = b1 - b0;
     }
     __syncthreads();
     for (uint32_t q0 = 0; q0 < a.nq; q0 += kStep) {
@@ -140,7 +194,7 @@ __global__ __launch_bounds__(kOrderSortThreads) void order_sort_kernel(OrderArgs
 #pragma unroll
             for (uint32_t u = 0; u < kOrderSortUnroll; u++) {
                 const uint32_t q = q0 + u * kOrderSortThreads + t;
-                k[u] = q < a.nq ? min(a.keys[q], kOrderPivots - 1u) : kNone;
+                k[u] = q < a.nq ? min(a.keys[q], nkeys - 1u) : kNone;
             }
         }
 #pragma unroll
@@ -1726,7 +1780,9 @@ hipError_t launch_order_pivots(const float *base, uint32_t dim_pad, uint32_t num
 
 hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream)
 {
-    if (a.key_dims == 0 || a.key_dims > kOrderKeyDims || a.key_dims > a.dim_pad || (a.key_dims & 1u) || a.nq == 0)
+    if (a.key_dims > kOrderKeyDims || a.key_dims > a.dim_pad || (a.key_dims & 1u) || a.nq == 0 || a.classes == 0 ||
+        a.classes > kMaxPoolClasses || !a.queues || (a.classes > 1 && (!a.Ls || !a.ranges)) ||
+        (a.key_dims && !a.pivots))
         return hipErrorInvalidValue;
     const uint32_t per_block = 2 * kOrderKeyWaves;  // queries per workgroup and pass
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((a.nq + per_block - 1) / per_block,

@@ -1310,6 +1310,15 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
     const uint32_t waves = blockDim.x >> 6;
     const size_t per_wave = wave_lds_bytes(a.dim_pad, a.l_max, hash_words(a), EW);
     const WaveLds w = carve(smem + per_wave * wave, a.dim_pad, a.l_max, EW);
+    // A pool-size class of a launch takes positions range[0] .. range[0] +
+    // range[1] of the order.
+    uint32_t rbeg = 0, rcnt = a.nq;
+    if (a.range) {
+        rbeg = uniform(a.range[0]);
+        rcnt = uniform(a.range[1]);
+        if (rbeg > a.nq || rcnt > a.nq - rbeg)
+            rcnt = 0;
+    }
     // One call site for the search body (two would keep it out of line).
     bool first = true;
     while (true) {
@@ -1319,9 +1328,9 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void search_kernel(S
             if (lane_id() == 0)
                 pos = atomicAdd(a.queue, 1u);
             pos = uniform(pos);
-            if (pos >= a.nq)
+            if (pos >= rcnt)
                 break;
-            q = a.order ? uniform(a.order[pos]) : pos;
+            q = a.order ? uniform(a.order[rbeg + pos]) : rbeg + pos;
             if (q >= a.nq)  // a malformed order entry: skip the position
                 continue;
         } else {

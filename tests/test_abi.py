@@ -83,10 +83,13 @@ def test_status_codes_match_the_header():
         assert getattr(_lib, name) == defs[name], name
     assert ctypes.sizeof(_lib.QueryStats) == 32 == STATS_DTYPE.itemsize
     assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
-    assert ctypes.sizeof(_lib.SearchParams) == 40 and _lib.SearchParams.query_order.offset == 28
+    assert ctypes.sizeof(_lib.SearchParams) == 44 and _lib.SearchParams.query_order.offset == 28
     assert _lib.SearchParams.team_size.offset == 32 and _lib.SearchParams.pool_layout.offset == 36
-    assert "uint32_t query_order;" in hdr and "uint32_t team_size;" in hdr and "uint32_t pool_layout;" in hdr
-    for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_NOALLOC_MAX_QUERIES"):
+    assert _lib.SearchParams.pool_classes.offset == 40
+    for f in ("query_order", "team_size", "pool_layout", "pool_classes"):
+        assert f"uint32_t {f};" in hdr, f
+    for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_NOALLOC_MAX_QUERIES", "SS_CLASSES_AUTO",
+                 "SS_CLASSES_OFF", "SS_CLASSES_SERIAL"):
         assert getattr(_lib, name) == defs[name], name
 
 

@@ -110,3 +110,64 @@ def test_autotune_reports_team_and_layout(c0):
         assert t["expand_width"] == 1 and t["waves_per_query"] == 1
         ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], q, 10, L)
         _same(ix.search(q, 10, L), ora, f"tuned L={L}")
+
+
+def test_pool_classes_equal_oracle_and_each_other(c0):
+    """A launch mixing K and L up to 512 is split by pool size (L <= 64,
+    <= 128, <= 256, <= 512) into kernels with their own LDS shapes, run side
+    by side, one after another, or not split: ids, distance bits, hops and
+    NDC equal the oracle in every mode, also when the launch is captured in
+    a graph and replayed, and through the no-allocation entry point."""
+    import ctypes
+    import torch
+    import oracle
+    from scattersearch_amd import dim_padded
+    from scattersearch_amd._lib import SS_CLASSES_AUTO, SS_CLASSES_OFF, SS_CLASSES_SERIAL, SearchParams, lib
+    base, adj, ep = c0["base"], c0["adj"], c0["ep"]
+    rng = np.random.default_rng(12)
+    nq = 600
+    q = base[rng.choice(base.shape[0], nq, replace=False)] + np.float32(0.05) * rng.standard_normal(
+        (nq, base.shape[1]), dtype=np.float32)
+    K = rng.choice([1, 10, 100], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(int(k), 513) for k in K], dtype=np.uint32)
+    ora = oracle.search_batch(base, adj, ep, q, K, L, k_stride=100)
+    ix = _index(base, adj, ep)
+
+    def check(ids, ds, st, what):
+        for i in range(nq):
+            k = int(K[i])
+            assert np.array_equal(ids[i, :k], ora[0][i, :k]), (what, i)
+            assert np.array_equal(ds[i, :k].view(np.uint32), ora[1][i, :k].view(np.uint32)), (what, i)
+        assert np.array_equal(st["hops"], ora[3]), what
+        clean = st["lossy"] == 0
+        assert np.array_equal(st["ndc"][clean], ora[2][clean]), what
+
+    from scattersearch_amd._lib import SS_ORDER_INDEX, SS_ORDER_LOCALITY
+    for mode in (SS_CLASSES_AUTO, SS_CLASSES_SERIAL, SS_CLASSES_OFF):
+        for rep in range(3 if mode == SS_CLASSES_AUTO else 1):
+            ids, ds, st = ix.search(q, K, L, k_stride=100, pool_classes=mode, use_graph=rep > 0)
+            check(ids, ds, st, f"mode {mode} rep {rep}")
+    # the classes taken longest pool first (the default), in no order, and
+    # by locality
+    for order in (SS_ORDER_INDEX, SS_ORDER_LOCALITY):
+        ids, ds, st = ix.search(q, K, L, k_stride=100, query_order=order)
+        check(ids, ds, st, f"order {order}")
+    # the no-allocation entry point on device buffers
+    dp = dim_padded(base.shape[1])
+    dq = torch.zeros((nq, dp), dtype=torch.float32, device="cuda")
+    dq[:, :base.shape[1]] = torch.from_numpy(q).cuda()
+    dK = torch.from_numpy(K.astype(np.int32)).cuda()
+    dL = torch.from_numpy(L.astype(np.int32)).cuda()
+    out = torch.zeros((nq, 100), dtype=torch.int32, device="cuda")
+    dist = torch.zeros((nq, 100), dtype=torch.float32, device="cuda")
+    stt = torch.zeros((nq, 8), dtype=torch.int32, device="cuda")
+    vp = ctypes.c_void_p
+    p = SearchParams()
+    rc = lib.ss_search_device_noalloc(ix.handle, vp(dq.data_ptr()), nq, vp(dK.data_ptr()), vp(dL.data_ptr()), 100,
+                                      512, 100, ctypes.byref(p), vp(out.data_ptr()), vp(dist.data_ptr()),
+                                      vp(stt.data_ptr()), None)
+    torch.cuda.synchronize()
+    assert rc == 0, lib.ss_last_error()
+    from scattersearch_amd.index import STATS_DTYPE
+    st = stt.cpu().numpy().view(np.uint32).reshape(-1).view(STATS_DTYPE)
+    check(out.cpu().numpy().view(np.uint32), dist.cpu().numpy(), st, "no-alloc")
