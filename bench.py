@@ -95,10 +95,12 @@ def main():
                     help="NSG file: load the graph from it if it exists, else build and write it")
     ap.add_argument("--profile", action="store_true",
                     help="only build, then run --steps batched launches (for rocprofv3)")
-    ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000",
+    ap.add_argument("--serve-rates", default="50000,100000,200000,300000,500000,800000,1000000",
                     help="Poisson arrival rates (QPS) of the serving sweep; empty to skip")
     ap.add_argument("--batch-sizes", default="1,64,1024,10000",
                     help="also time hipGraph-replayed launches of these batch sizes; empty to skip")
+    ap.add_argument("--serve-config", default="",
+                    help='serving knobs as JSON, e.g. {"standby_waves": 1024} (ss_serve_config fields)')
     ap.add_argument("--serve-seconds", type=float, default=0.5,
                     help="length of the arrival stream at each serving rate (at least 20,000 requests)")
     ap.add_argument("--gt-queries", type=int, default=-1,
@@ -356,7 +358,7 @@ def main():
     if args.serve_rates:
         try:
             serving = serve_sweep(ix, queries, gt, w, [float(r) for r in args.serve_rates.split(",")],
-                                  args.serve_seconds)
+                                  args.serve_seconds, json.loads(args.serve_config) if args.serve_config else None)
         except Exception as exc:  # the batch numbers above stay valid
             serving = {"error": str(exc)}
             log(f"serving sweep failed: {exc}")
@@ -466,11 +468,11 @@ def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_
     return out
 
 
-def serve_sweep(ix, queries, gt10, w, rates, seconds):
+def serve_sweep(ix, queries, gt10, w, rates, seconds, cfg=None):
     """Online serving (the C4 workload): Poisson arrivals at each rate into
     the serving loop for `seconds` (at least 20,000 requests), K = 10 (80%)
     or 100 (20%), L uniform in max(64, K)..512; latency = completion seen by
-    the host minus arrival."""
+    the host minus arrival.  cfg: ss_serve_config fields to set."""
     from scattersearch_amd import metrics, serve
     out = []
     for r in rates:
@@ -481,7 +483,8 @@ def serve_sweep(ix, queries, gt10, w, rates, seconds):
         L = rng.integers(np.maximum(64, K), 513).astype(np.uint32)  # K <= L
         arr = serve.poisson_arrivals(nreq, r, seed=int(r))
         t = time.perf_counter()
-        ids, ds, lat, st = serve.replay(ix, queries[qi], K, L, arr, k_max=100, l_max=512)
+        ids, ds, lat, st, tr = serve.replay(ix, queries[qi], K, L, arr, k_max=100, l_max=512, with_trace=True,
+                                            **(cfg or {}))
         wall = time.perf_counter() - t
         span_s = (arr[-1] + lat[-1]) * 1e-6
         khz = ix.wall_clock_khz()
@@ -494,6 +497,14 @@ def serve_sweep(ix, queries, gt10, w, rates, seconds):
                "service_p50_ms": round(metrics.percentile(svc, 0.5), 4),
                "service_p99_ms": round(metrics.percentile(svc, 0.99), 4),
                "recall@10": round(rec, 4), "mean_ndc": round(float(st["ndc"].mean()), 1), "wall_s": round(wall, 2)}
+        # where the time went (host clock; device times mapped by a clock reading)
+        parts = {"submit_lag": tr["submit_us"] - arr, "claim_wait": tr["claim_us"] - tr["submit_us"],
+                 "notify": tr["seen_us"] - tr["finish_us"]}
+        row["split_ms"] = {k: [round(metrics.percentile(v, 0.5) / 1e3, 4), round(metrics.percentile(v, 0.99) / 1e3, 4)]
+                           for k, v in parts.items()}
+        busy = tr["waves_alive"].astype(np.int64) - tr["waves_standby"]
+        row["waves"] = {"alive_p50": int(np.median(tr["waves_alive"])), "alive_min": int(tr["waves_alive"].min()),
+                        "busy_p50": int(np.median(busy)), "busy_max": int(busy.max())}
         out.append(row)
         log(f"serve {row}")
     return {"workload": "serving kernel with continuous admission (wavefronts claim requests from a host ring "

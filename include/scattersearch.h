@@ -322,15 +322,29 @@ int ss_poll(ss_server *server, uint32_t max_results, uint64_t *tickets, uint32_t
  * freed; unpolled results and unclaimed requests are dropped. */
 int ss_serve_stop(ss_server *server);
 
+/* Where a replayed request's time went: microseconds since the replay
+ * started on the host clock.  Device times (stats.t_start / t_end) are mapped
+ * to the host clock through a calibration read of the device wall clock,
+ * good to a few microseconds. */
+typedef struct ss_serve_trace {
+    double submit_us; /* ss_submit published it (arrival_us[i] or later) */
+    double claim_us;  /* a wavefront took it */
+    double finish_us; /* its results were written */
+    double seen_us;   /* the poller saw its done flag */
+    uint32_t waves_alive;   /* wavefronts in the server's activations, and of */
+    uint32_t waves_standby; /* them those waiting for work, when it was seen */
+} ss_serve_trace;
+
 /* Open-loop load test: starts a server, submits query i at arrival_us[i]
  * microseconds after the start from one thread while another polls, and
  * returns each result with its latency (completion seen by the poller minus
  * arrival_us[i], the definition of tests/online_parallel_search.cpp:154-158)
- * in out_ids/out_dists [nq][k_stride] and latency_us [nq]. */
+ * in out_ids/out_dists [nq][k_stride] and latency_us [nq].  stats and trace
+ * [nq] may be NULL. */
 int ss_serve_replay(ss_index *index, const ss_serve_config *config, const float *queries,
                     uint32_t nq, const uint32_t *K, const uint32_t *L, const double *arrival_us,
                     uint32_t k_stride, uint32_t *out_ids, float *out_dists, double *latency_us,
-                    ss_query_stats *stats);
+                    ss_query_stats *stats, ss_serve_trace *trace);
 
 /* ---------------------------------------------------------------------- */
 /* Host-side files and graph construction (no GPU needed).                  */

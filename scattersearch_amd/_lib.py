@@ -151,7 +151,7 @@ _SIGS = {
     "ss_serve_stop": (cint, [vp]),
     "ss_serve_replay": (cint, [vp, ctypes.POINTER(ServeConfig), f32p, u32, u32p, u32p,
                                ctypes.POINTER(ctypes.c_double), u32, u32p, f32p,
-                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(QueryStats)]),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(QueryStats), vp]),
     "ss_fbin_header": (cint, [cstr, u32p, u32p]),
     "ss_fbin_read": (cint, [cstr, f32p, u32, u32]),
     "ss_fbin_write": (cint, [cstr, f32p, u32, u32]),

@@ -93,13 +93,17 @@ struct OrderArgs {
 // host memory, search them with the exact kernel body and publish each
 // result on its own.  search.queries/Ks/Ls/out_ids/out_dists/stats point
 // into the ring (indexed by ring slot); search.k_stride is the ring's k_max.
-struct ServeCtl {        // device memory, one 64-byte block per server
-    uint64_t head;       // requests claimed
+struct ServeCtl {        // device memory, two 64-byte lines per server
+    int64_t avail;       // requests published (as far as seen) and not yet claimed
     uint64_t seen_tail;  // largest `tail` read from host memory so far
     uint64_t refreshed;  // wall clock of the last read of host memory
     uint32_t standby;    // wavefronts waiting for work
     uint32_t stop;       // copy of the host's stop flag
-    uint64_t pad[4];
+    uint32_t alive;      // wavefronts in running activations
+    uint32_t pad32;
+    uint64_t pad0[3];
+    uint64_t head;       // tickets handed out (own line: every claim adds to it)
+    uint64_t pad1[7];
 };
 
 struct ServeArgs {
@@ -109,6 +113,7 @@ struct ServeArgs {
     ServeCtl *ctl;          // device
     uint64_t *claimed;      // host: per slot, ticket + 1 once a wavefront owns it
     uint64_t *done;         // host: per slot, ticket + 1 once its result is written
+    uint32_t *mirror;       // host: alive and standby wavefronts, copied at each read of tail
     uint32_t mask;          // ring slots - 1
     uint32_t max_idle_polls;
     uint32_t max_standby;   // idle wavefronts kept per server; others leave
@@ -217,6 +222,9 @@ hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream
 // what: 0 = random initial lists, 1 = one pull iteration
 hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, hipStream_t stream);
 hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream);
+// Writes the device wall clock (the counter behind ss_query_stats' times) to
+// device memory at `out`.
+hipError_t launch_read_clock(uint64_t *out, hipStream_t stream);
 hipError_t launch_clear_u32(uint32_t *p, size_t n, hipStream_t stream);
 size_t nnd_lds_bytes_host(uint32_t dim_pad, uint32_t hash_slots);
 hipError_t launch_bruteforce(int metric, const BruteArgs &a, hipStream_t stream);

@@ -709,22 +709,22 @@ __global__ __launch_bounds__(512, scatter_min_waves<NBLK>()) void scatter_kernel
 // ---------------------------------------------------------------------------
 // Serving activation.  Requests are published by the host (ring slots filled,
 // then `tail` raised in host memory).  A wavefront claims the next request by
-// advancing ctl.head with a compare-and-swap against the largest tail seen so
-// far (ctl.seen_tail, device memory), runs the exact search on it and
-// publishes its result with a per-slot flag, so every request completes on
-// its own.  Host memory is read for new requests at most once per
+// taking one from ctl.avail (requests seen in host memory and not yet claimed)
+// and drawing a ticket from ctl.head (claim_ticket), runs the exact search on
+// it and publishes its result with a per-slot flag, so every request completes
+// on its own.  Host memory is read for new requests at most once per
 // refresh_ticks across the whole activation: a wavefront that finds no work
 // takes the refresh turn with a compare-and-swap on ctl.refreshed and copies
-// the host's tail (and stop flag) into ctl.  At most max_standby wavefronts
+// the host's tail (and stop flag) into ctl, adding the newly seen requests to
+// ctl.avail.  At most max_standby wavefronts
 // wait for work at a time (counted in ctl.standby); the others leave when
 // idle.  The n-th standby wavefront polls ctl every 1 + n/64 sleeps, so the
 // standby pool as a whole polls a few tens of millions of times a second
 // however large it is.  A standby wavefront leaves after linger_ticks of wall
 // clock (or max_idle_polls polls) without work, or when the host's stop flag
 // is seen.
-// No wavefront waits for another and every loop is bounded: the claim loop by
-// the number of published requests (a failed swap means another wavefront
-// claimed one), the idle loop by the poll count.
+// No wavefront waits for another and every loop is bounded: a claim is a
+// fixed number of atomic adds, the idle loop is bounded by the poll count.
 //
 // Ordering: every word of the ring (request vector, K, L, result ids and
 // distances, stats, claim and done flags) is read and written with
@@ -738,15 +738,21 @@ constexpr uint64_t kNoTicket = ~0ull;
 
 __device__ __forceinline__ uint64_t claim_ticket(ServeCtl *ctl)
 {
-    uint64_t h = __hip_atomic_load(&ctl->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (true) {
-        const uint64_t t = __hip_atomic_load(&ctl->seen_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h >= t)
-            return kNoTicket;
-        if (__hip_atomic_compare_exchange_strong(&ctl->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return h;
+    // A wavefront that takes one request from `avail` (positive before its
+    // decrement) draws the next ticket from `head`.  Takes never outnumber
+    // the requests seen, so every ticket drawn is published; a take that
+    // found nothing is given back.  Atomic adds only: a compare-and-swap
+    // loop on `head` let hundreds of wavefronts retry against each other
+    // for every ticket.
+    if (__hip_atomic_load(&ctl->avail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+        return kNoTicket;
+    const int64_t before = __hip_atomic_fetch_add(&ctl->avail, (int64_t)-1, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    if (before <= 0) {
+        __hip_atomic_fetch_add(&ctl->avail, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return kNoTicket;
     }
+    return __hip_atomic_fetch_add(&ctl->head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One read of the host's tail and stop flag, if this wavefront wins the
@@ -762,9 +768,17 @@ __device__ __forceinline__ void refresh_from_host(const ServeArgs &a)
         return;
     const uint64_t t = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint32_t stop = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_fetch_max(&a.ctl->seen_tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t seen = __hip_atomic_fetch_max(&a.ctl->seen_tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t > seen)  // each newly seen request becomes claimable once
+        __hip_atomic_fetch_add(&a.ctl->avail, (int64_t)(t - seen), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (stop)
         __hip_atomic_store(&a.ctl->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the host's view of the wavefront counts (for load tracing)
+    __hip_atomic_store(a.mirror, __hip_atomic_load(&a.ctl->alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(a.mirror + 1,
+                       __hip_atomic_load(&a.ctl->standby, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ __forceinline__ uint64_t uniform64(uint64_t v)
@@ -786,6 +800,8 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(Se
     bool standby = false;  // counted in ctl.standby
     uint64_t idle_since = 0;
     uint32_t polls = 0, sleeps = 1;
+    if (lane == 0)
+        __hip_atomic_fetch_add(&a.ctl->alive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (true) {
         uint64_t tk = kNoTicket;
         if (lane == 0) {
@@ -838,6 +854,8 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(Se
         for (uint32_t i = 0; i < sleeps; i++)
             __builtin_amdgcn_s_sleep(127);  // about 8,000 cycles each
     }
+    if (lane == 0)
+        __hip_atomic_fetch_add(&a.ctl->alive, 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1839,6 +1857,18 @@ hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, h
 hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream)
 {
     hipLaunchKernelGGL(nnd_reverse_kernel, dim3(2048), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+__global__ void read_clock_kernel(uint64_t *out)
+{
+    if (threadIdx.x == 0)
+        *out = (uint64_t)wall_clock64();
+}
+
+hipError_t launch_read_clock(uint64_t *out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(64), 0, stream, out);
     return hipGetLastError();
 }
 

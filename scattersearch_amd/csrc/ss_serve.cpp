@@ -51,9 +51,10 @@ using clk = std::chrono::steady_clock;
 
 // The part of the ring the GPU reads or writes, in one coherent allocation.
 struct Control {
-    uint64_t tail;  // requests published
-    uint32_t stop;  // nonzero: idle wavefronts leave at once
-    uint32_t pad[13];
+    uint64_t tail;        // requests published
+    uint32_t stop;        // nonzero: idle wavefronts leave at once
+    uint32_t mirror[2];   // alive and standby wavefronts, as the GPU last wrote them
+    uint32_t pad[11];
 };
 
 template <typename T>
@@ -205,6 +206,46 @@ static void dispatch_loop(ss_server *s)
     }
 }
 
+// One reading of the device wall clock against the host clock: of five
+// reads, the one whose bracket (host time before the launch to after its
+// completion) was shortest, taken at the bracket's midpoint.
+template <typename NowUs>
+static int device_clock_pair(int device, NowUs now_us, double *host_us, uint64_t *dev_ticks)
+{
+    hipError_t e = hipSetDevice(device);
+    uint64_t *d = nullptr;
+    hipStream_t st = nullptr;
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&d), sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    double best = 1e300;
+    for (int i = 0; i < 5 && e == hipSuccess; i++) {
+        const double a = now_us();
+        e = ss::launch_read_clock(d, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        const double b = now_us();
+        uint64_t ticks = 0;
+        if (e == hipSuccess)
+            e = hipMemcpy(&ticks, d, sizeof(ticks), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && b - a < best) {
+            best = b - a;
+            *host_us = 0.5 * (a + b);
+            *dev_ticks = ticks;
+        }
+    }
+    if (st)
+        (void)hipStreamDestroy(st);
+    if (d)
+        (void)hipFree(d);
+    if (e != hipSuccess) {
+        set_error(hip_msg(e, "ss_serve_replay: device clock reading"));
+        return SS_EDEVICE;
+    }
+    return SS_OK;
+}
+
 extern "C" {
 
 int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
@@ -332,6 +373,7 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     a.ctl = s->d_ctl;
     a.claimed = dev_ptr(s->h_claimed);
     a.done = dev_ptr(s->h_done);
+    a.mirror = dev_ptr(s->h_ctl->mirror);
     a.mask = s->mask;
     a.max_idle_polls = kMaxIdlePolls;
     a.max_standby = c.standby_waves;
@@ -343,7 +385,7 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     a.linger_ticks = (uint64_t)khz * c.linger_us / 1000u;
     a.refresh_ticks = std::max<uint64_t>(1, (uint64_t)khz * kRefreshNs / 1000000u);
     if (!a.search.queries || !a.search.Ks || !a.search.Ls || !a.search.out_ids || !a.search.out_dists ||
-        !a.search.stats || !a.tail || !a.stop || !a.claimed || !a.done || a.linger_ticks == 0) {
+        !a.search.stats || !a.tail || !a.stop || !a.claimed || !a.done || !a.mirror || a.linger_ticks == 0) {
         set_error("ss_serve_start: cannot map the request ring for the GPU");
         free_server(s);
         return SS_EDEVICE;
@@ -456,7 +498,8 @@ int ss_serve_stop(ss_server *s)
 
 int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *queries, uint32_t nq,
                     const uint32_t *K, const uint32_t *L, const double *arrival_us, uint32_t k_stride,
-                    uint32_t *out_ids, float *out_dists, double *latency_us, ss_query_stats *stats)
+                    uint32_t *out_ids, float *out_dists, double *latency_us, ss_query_stats *stats,
+                    ss_serve_trace *trace)
 {
     if (!ix || !config || !queries || !K || !L || !arrival_us || !out_ids || !latency_us) {
         set_error("ss_serve_replay: NULL argument");
@@ -472,15 +515,25 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
     int rc = ss::index_view(ix, &v);
     if (rc)
         return rc;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now_us = [&]() { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); };
+    double clock_host_us = 0.0, us_per_tick = 0.0;
+    uint64_t clock_dev = 0;
+    if (trace) {
+        uint32_t khz = 0;
+        if ((rc = ss_wall_clock_khz(ix, &khz)) || (rc = device_clock_pair(v.device, now_us, &clock_host_us, &clock_dev)))
+            return rc;
+        us_per_tick = 1000.0 / khz;
+    }
+    auto dev_us = [&](uint64_t ticks) { return clock_host_us + ((double)ticks - (double)clock_dev) * us_per_tick; };
+    std::vector<double> submit_us(trace ? nq : 0);
     ss_server *s = nullptr;
     if ((rc = ss_serve_start(ix, config, &s)))
         return rc;
-    using clk = std::chrono::steady_clock;
     std::atomic<int> prod_rc{SS_OK};
     std::string prod_msg;  // written by the producer before it stores prod_rc
     std::atomic<bool> abandon{false};
-    const auto t0 = clk::now();
-    auto now_us = [&]() { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); };
     std::thread producer([&]() {
         for (uint32_t i = 0; i < nq && !abandon.load(); i++) {
             while (now_us() < arrival_us[i] && !abandon.load()) {
@@ -495,6 +548,8 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
                 prod_rc.store(r);
                 return;
             }
+            if (trace)
+                submit_us[i] = now_us();
         }
     });
     const uint32_t batch = 256;
@@ -517,6 +572,10 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
                 std::memcpy(out_dists + i * k_stride, ds.data() + (size_t)j * config->k_max, config->k_max * 4);
             if (stats)
                 stats[i] = st[j];
+            if (trace)
+                trace[i] = ss_serve_trace{0.0, dev_us(st[j].t_start), dev_us(st[j].t_end), t,
+                                          __atomic_load_n(&s->h_ctl->mirror[0], __ATOMIC_RELAXED),
+                                          __atomic_load_n(&s->h_ctl->mirror[1], __ATOMIC_RELAXED)};
         }
         done += n;
         if (n == 0 && t > give_up_us) {
@@ -527,6 +586,9 @@ int ss_serve_replay(ss_index *ix, const ss_serve_config *config, const float *qu
     }
     abandon.store(true);
     producer.join();
+    if (trace)
+        for (uint32_t i = 0; i < nq; i++)
+            trace[i].submit_us = submit_us[i];  // the producer has joined
     if (!rc && prod_rc.load() != SS_OK) {
         rc = prod_rc.load();
         set_error(prod_msg);  // the message was recorded on the producer thread

@@ -70,9 +70,18 @@ class Server:
             pass
 
 
-def replay(index, queries, K, L, arrival_us, k_max=None, l_max=None, **kw):
+# ss_serve_trace: where a replayed request's time went (host clock, us since
+# the start) and the server's wavefront counts when its result was seen
+TRACE_DTYPE = np.dtype([("submit_us", "<f8"), ("claim_us", "<f8"), ("finish_us", "<f8"), ("seen_us", "<f8"),
+                        ("waves_alive", "<u4"), ("waves_standby", "<u4")])
+
+
+def replay(index, queries, K, L, arrival_us, k_max=None, l_max=None, with_trace=False, **kw):
     """Open-loop load test: query i is submitted arrival_us[i] microseconds
-    after the start.  Returns (ids [nq][k_max], dists, latency_us, stats)."""
+    after the start.  Returns (ids [nq][k_max], dists, latency_us, stats),
+    and with_trace=True adds each request's timeline (TRACE_DTYPE: submit,
+    claim, finish and seen times in microseconds on the host clock since the
+    start, and the server's wavefront counts when it was seen)."""
     q = np.ascontiguousarray(queries, dtype=np.float32)
     nq = q.shape[0]
     Ka = np.ascontiguousarray(np.broadcast_to(np.asarray(K, np.uint32), (nq,)))
@@ -85,12 +94,14 @@ def replay(index, queries, K, L, arrival_us, k_max=None, l_max=None, **kw):
     ds = np.empty((nq, k_max), dtype=np.float32)
     lat = np.empty(nq, dtype=np.float64)
     st = np.zeros(nq, dtype=STATS_DTYPE)
+    tr = np.zeros(nq, dtype=TRACE_DTYPE) if with_trace else None
     dp = ctypes.POINTER(ctypes.c_double)
     check(lib.ss_serve_replay(index.handle, ctypes.byref(cfg), fptr(q), nq, uptr(Ka), uptr(La),
                               arr.ctypes.data_as(dp), k_max, uptr(ids), fptr(ds), lat.ctypes.data_as(dp),
-                              st.ctypes.data_as(ctypes.POINTER(QueryStats))))
+                              st.ctypes.data_as(ctypes.POINTER(QueryStats)),
+                              tr.ctypes.data_as(ctypes.c_void_p) if with_trace else None))
     _check_status(st)
-    return ids, ds, lat, st
+    return (ids, ds, lat, st, tr) if with_trace else (ids, ds, lat, st)
 
 
 def _check_status(st):

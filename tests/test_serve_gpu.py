@@ -55,9 +55,17 @@ def test_serving_results_equal_alone_at_the_serving_mix(small):
     L = rng.integers(np.maximum(64, K), 513).astype(np.uint32)
     arr = serve.poisson_arrivals(nq, 50000, seed=3)
     print(f"serving mix: {nq} requests, K in {{10, 100}}, L {L.min()}..{L.max()}", flush=True)
-    ids, ds, lat, st = serve.replay(ix, q, K, L, arr, k_max=100, l_max=512)
+    ids, ds, lat, st, tr = serve.replay(ix, q, K, L, arr, k_max=100, l_max=512, with_trace=True)
     assert (lat > 0).all()
     assert (st["status"] == 0).all()
+    # the timeline of every request is in order (device times are mapped to
+    # the host clock through one reading, good to some tens of microseconds)
+    submit, claim, finish, seen = tr["submit_us"], tr["claim_us"], tr["finish_us"], tr["seen_us"]
+    assert (submit >= arr).all() and (seen > submit).all()
+    np.testing.assert_allclose(seen - arr, lat, atol=1e-6)
+    assert (finish >= claim).all()
+    assert (claim > submit - 100.0).all() and (finish < seen + 100.0).all()
+    assert (tr["waves_alive"] >= tr["waves_standby"]).all() and tr["waves_alive"].max() > 0
     o_ids, o_d, o_ndc, o_hops = oracle.search_batch(small["base"], small["adj"], small["ep"], q, K, L, k_stride=100)
     for i in range(nq):
         k = int(K[i])
