@@ -385,7 +385,7 @@ def main():
                    "mean_ndc": round(float(ndc_exact.mean()), 1), "mean_hops": round(float(st["hops"].mean()), 1),
                    "recomputed_ndc": round(float((st["ndc"] - ndc_exact).mean()), 1),
                    "launch": dict({k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots",
-                                                         "team_size", "pool_layout")},
+                                                         "visited_bucket", "team_size", "pool_layout")},
                                   query_order=order_used(nq))},
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
@@ -632,7 +632,8 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
                    "K_mix": {str(k): int((Ks == k).sum()) for k in (1, 10, 100)}, "L_mean": float(Ls.mean()),
                    "launch": {"pool_classes": "concurrent",
                               "classes": {f"L<={b}": {k: t[k] for k in ("waves_per_block", "visited_slots",
-                                                                          "team_size", "pool_layout")}
+                                                                          "visited_bucket", "team_size",
+                                                                          "pool_layout")}
                                           for b, t in tuned_cls.items()}}},
         "pool_classes_kernel_ms": variants,
         "recall": rec,

@@ -109,6 +109,13 @@ typedef struct ss_search_params {
                                  kernels one after another; SS_CLASSES_OFF
                                  (1): one kernel sized for the launch's
                                  largest L */
+    uint32_t visited_bucket;  /* slots per bucket of the 16-bit visited table
+                                 (one wavefront per query): 8 (one 16-byte
+                                 LDS read per lookup) or 16 (two reads, far
+                                 fewer full buckets, so the same ids fit a
+                                 smaller table and more wavefronts fit a CU);
+                                 0 = 8.  Results do not depend on it (a full
+                                 bucket only recomputes a distance) */
 } ss_search_params;
 
 #define SS_CLASSES_AUTO 0u
@@ -299,7 +306,7 @@ typedef struct ss_serve_config {
     uint32_t ring_slots;      /* most requests in flight (power of two); 0 = 16384 */
     uint32_t visited_slots;   /* per-query visited hash; 0 = tuned or default */
     uint32_t waves_per_block; /* 0 = tuned or default */
-    uint32_t blocks;          /* blocks per activation; 0 = what the LDS holds, <= 8 waves per CU */
+    uint32_t blocks;          /* blocks per activation; 0 = what the LDS holds, <= 4 waves per CU */
     uint32_t streams;         /* activations alive at once, 1..8; 0 = 2 */
     uint32_t linger_us;       /* how long a standby wavefront waits for work, <= 200000; 0 = 20000 */
     uint32_t admit_us;        /* an unclaimed request this old starts another activation; 0 = 50 */

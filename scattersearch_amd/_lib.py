@@ -57,6 +57,7 @@ class SearchParams(ctypes.Structure):
         ("team_size", ctypes.c_uint32),
         ("pool_layout", ctypes.c_uint32),
         ("pool_classes", ctypes.c_uint32),
+        ("visited_bucket", ctypes.c_uint32),
     ]
 
 

@@ -126,7 +126,8 @@ struct ss_index {
     // Launch parameters chosen by ss_autotune, per pool-size bucket
     // (next power of two of L); dim and degree are fixed per index.
     struct Tuned {
-        uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query, team_size, pool_layout;
+        uint32_t l_bucket, expand_width, waves_per_block, visited_slots, waves_per_query, team_size, pool_layout,
+            visited_bucket;
     };
     std::vector<Tuned> tuned;
     // Streams and events that let the kernels of a launch's pool-size
@@ -518,7 +519,7 @@ int ss_load_graph(ss_index *ix, const char *path)
 struct Shape {
     uint32_t wpb, slots, blocks;
     size_t lds_wave;
-    bool hash16;  // 16-bit visited slots (one wavefront per query)
+    uint32_t hash16;  // 16-bit visited slots (one wavefront per query): 1 buckets of 8, 2 of 16
     uint32_t hash_rem_bits, hash_id_mask;
     uint32_t team, pool_lds;
 };
@@ -546,6 +547,8 @@ static ss_search_params resolve_params(const ss_index *ix, uint32_t l_max, const
             r.team_size = t.team_size;
         if (!r.pool_layout)
             r.pool_layout = t.pool_layout;
+        if (!r.visited_bucket)
+            r.visited_bucket = t.visited_bucket;
     }
     if (!r.expand_width)
         r.expand_width = 1;
@@ -586,6 +589,11 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         set_error("team_size 8 needs expand_width 1 and one wavefront per query");
         return SS_EINVAL;
     }
+    const uint32_t bucket = p && p->visited_bucket ? p->visited_bucket : 8;
+    if (bucket != 8 && bucket != 16) {
+        set_error("visited_bucket must be 8 or 16");
+        return SS_EINVAL;
+    }
     const uint32_t layout = p ? p->pool_layout : SS_POOL_AUTO;
     if (layout > SS_POOL_LDS || (layout == SS_POOL_REGISTERS && l_max > 64)) {
         set_error("pool_layout must be SS_POOL_AUTO, SS_POOL_REGISTERS (L <= 64) or SS_POOL_LDS");
@@ -607,21 +615,23 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         s->slots = slots;
         s->blocks = blocks;
         s->lds_wave = per_block / wq;
-        s->hash16 = false;
+        s->hash16 = 0;
         s->hash_rem_bits = s->hash_id_mask = 0;
         return SS_OK;
     }
     s->hash_rem_bits = s->hash_id_mask = 0;
     s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu,
                                ss::search_reg_waves_per_cu(ix->metric, ix->dim_pad, ew, l_max, team, s->pool_lds),
-                               &s->hash_rem_bits, &s->hash_id_mask);
+                               bucket, &s->hash_rem_bits, &s->hash_id_mask);
     const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots, ew);
     const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
     if (waves_cu == 0) {
         set_error("search state does not fit in LDS: lower L or visited_slots");
         return SS_EINVAL;
     }
-    uint32_t wpb = p && p->waves_per_block ? p->waves_per_block : std::min<uint32_t>(waves_cu, 4);
+    // default: the largest of 4, 2, 1 wavefronts per block that divides the
+    // wavefronts a CU holds, so that whole blocks use all of them
+    uint32_t wpb = p && p->waves_per_block ? p->waves_per_block : (waves_cu % 4 == 0 ? 4u : (waves_cu % 2 == 0 ? 2u : 1u));
     if (wpb == 0 || wpb > 8 || wpb > waves_cu) {
         set_error("waves_per_block must be 1..8 and fit in LDS");
         return SS_EINVAL;
@@ -640,14 +650,15 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
 }  // extern "C"
 
 namespace ss {
-bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *id_mask)
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask)
 {
     uint32_t sb = 0;  // log2(slots)
     while ((1u << sb) < slots)
         sb++;
-    if (sb < 3)
+    const uint32_t lb = bucket == 16 ? 4u : 3u;  // log2(slots per bucket)
+    if (sb < lb)
         return false;
-    const uint32_t bb = sb - 3;  // buckets of eight slots
+    const uint32_t bb = sb - lb;  // log2(buckets)
     uint32_t n = 0;  // ceil(log2(num))
     while (n < 32 && (1ull << n) < num)
         n++;
@@ -660,18 +671,20 @@ bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *i
     return true;
 }
 
-bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
-                size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t *rem_bits, uint32_t *id_mask)
+uint32_t use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
+                    size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask)
 {
     if (expand_width != 1)  // the wider-expansion kernels keep the 32-bit table
-        return false;
+        return 0;
+    if (bucket == 16)  // asked for by name: the 16-bit table wherever the ids fit it
+        return hash_params16(num, slots, 16, rem_bits, id_mask) ? 2u : 0u;
     const size_t lds_cu = lds_per_cu ? lds_per_cu : 65536;
     const size_t w32 = std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots, 1), reg_waves_cu);
     const size_t w16 =
         std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots / 2, 1), reg_waves_cu);
     if (w16 <= w32)
-        return false;
-    return hash_params16(num, slots, rem_bits, id_mask);
+        return 0;
+    return hash_params16(num, slots, 8, rem_bits, id_mask) ? 1u : 0u;
 }
 
 int index_view(const ss_index *ix, IndexView *v)
@@ -810,7 +823,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     a.l_max = l_max;
     a.hash_slots = sh.slots;
     a.hash_shift = 32u - log2u(sh.slots);
-    a.hash16 = sh.hash16 ? 1u : 0u;
+    a.hash16 = sh.hash16;
     a.hash_rem_bits = sh.hash_rem_bits;
     a.hash_id_mask = sh.hash_id_mask;
     a.k_stride = k_stride;
@@ -912,7 +925,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             ac.l_max = bound;
             ac.hash_slots = shc.slots;
             ac.hash_shift = 32u - log2u(shc.slots);
-            ac.hash16 = shc.hash16 ? 1u : 0u;
+            ac.hash16 = shc.hash16;
             ac.hash_rem_bits = shc.hash_rem_bits;
             ac.hash_id_mask = shc.hash_id_mask;
             ac.team = shc.team;
@@ -1581,6 +1594,7 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                 p.waves_per_query = 1;
                 p.team_size = 4;
                 p.pool_layout = dflt_layout;
+                p.visited_bucket = 8;
                 cands.push_back(p);
             }
     }
@@ -1672,7 +1686,8 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                                    [&](const ss_index::Tuned &t) { return t.l_bucket == bucket; }),
                     ix->tuned.end());
     ix->tuned.push_back(ss_index::Tuned{bucket, best->p.expand_width, best->p.waves_per_block, best->p.visited_slots,
-                                        best->p.waves_per_query, best->p.team_size, best->p.pool_layout});
+                                        best->p.waves_per_query, best->p.team_size, best->p.pool_layout,
+                                        best->p.visited_bucket});
     if (best_out)
         *best_out = best->p;
     if (best_ms_out)

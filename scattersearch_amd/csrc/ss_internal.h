@@ -29,7 +29,8 @@ struct SearchArgs {
     uint32_t l_max;       // sizes the LDS pool (<= SS_MAX_L)
     uint32_t hash_shift;  // 32 - log2(hash slots)
     uint32_t hash_slots;
-    uint32_t hash16;        // 1: 16-bit visited slots (hash_params16), one wavefront per query only
+    uint32_t hash16;        // 16-bit visited slots (hash_params16), one wavefront per query only:
+                            // 1 in buckets of 8 slots, 2 in buckets of 16; 0: 32-bit slots
     uint32_t hash_rem_bits; // 16-bit slots: bits of the hashed id kept in a slot (<= 15)
     uint32_t hash_id_mask;  // 16-bit slots: ids are hashed modulo this + 1 (a power of two)
     uint32_t k_stride;
@@ -280,19 +281,22 @@ struct IndexView {
     bool has_graph;
 };
 int index_view(const ss_index *ix, IndexView *v);
-// Whether a visited hash of `slots` slots (buckets of eight) can hold the ids
-// of an index of `num` points in 16-bit slots: with b = log2(slots / 8) and
-// n = max(ceil(log2(num)), b), id -> (id * 0x9E3779B1) mod 2^n is a bijection
-// whose top b bits pick the bucket and whose low n - b bits are stored, which
-// needs n - b <= 15 (bit 15 marks an empty slot).
-bool hash_params16(uint32_t num, uint32_t slots, uint32_t *rem_bits, uint32_t *id_mask);
-// Whether a one-wavefront-per-query launch (one node per step) uses 16-bit
-// slots: when they fit (hash_params16) and keep more wavefronts resident
-// per CU than the 32-bit table, given the LDS of a CU and reg_waves_cu, the
-// wavefronts per CU the kernel's registers allow; otherwise the 32-bit
-// table, whose lookup and claim take fewer instructions and LDS round trips.
-bool use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
-                size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t *rem_bits, uint32_t *id_mask);
+// Whether a visited hash of `slots` slots (buckets of `bucket` = 8 or 16
+// slots) can hold the ids of an index of `num` points in 16-bit slots: with
+// b = log2(slots / bucket) and n = max(ceil(log2(num)), b), id ->
+// (id * 0x9E3779B1) mod 2^n is a bijection whose top b bits pick the bucket and
+// whose low n - b bits are stored, which needs n - b <= 15 (bit 15 marks an
+// empty slot).
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask);
+// Which visited table a one-wavefront-per-query launch (one node per step)
+// uses: 2 (16-bit slots in buckets of 16) when bucket = 16 asks for it and the
+// ids fit (hash_params16); with bucket = 8, 1 (16-bit slots in buckets of 8)
+// when they fit and keep more wavefronts resident per CU than the 32-bit
+// table, given the LDS of a CU and reg_waves_cu, the wavefronts per CU the
+// kernel's registers allow; otherwise 0, the 32-bit table, whose lookup and
+// claim take fewer instructions and LDS round trips.
+uint32_t use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
+                    size_t lds_per_cu, uint32_t reg_waves_cu, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask);
 // Wavefronts per CU the registers of the one-wavefront-per-query search
 // kernel for (metric, dim_pad, expand_width, pool bound, team, pool layout)
 // allow.

@@ -426,46 +426,71 @@ __device__ __forceinline__ bool visit(uint32_t *tab, uint32_t bshift, uint32_t i
 }
 
 // The same set in 16-bit slots, for a table only one wavefront writes (one
-// wavefront per query): a bucket of eight slots is one 16-byte LDS read.
-// h = (id * 0x9E3779B1) & id_mask is a bijection of the ids; the bucket is
-// h >> rem_bits and the slot keeps h's low rem_bits (<= 15) bits, so a slot
-// value and its bucket give the id back exactly.  An empty slot is 0xFFFF,
-// the only value with bit 15 set.  A claim is a plain 16-bit store; lanes of
-// one step that stored into the same slot are sorted out by reading the slot
-// back in visit16_settle (the last store wins, the others move on).
-__device__ __forceinline__ bool visit16_begin(uint32_t *tab, uint32_t rem_bits, uint32_t id_mask, uint32_t id,
-                                              VisitClaim &c, uint32_t &lossy)
+// wavefront per query): a bucket of eight slots is one 16-byte LDS read, a
+// wide bucket of sixteen slots two (far fewer full buckets at the same load,
+// so a smaller table serves a pool).  h = (id * 0x9E3779B1) & id_mask is a
+// bijection of the ids; the bucket is h >> rem_bits and the slot keeps h's low
+// rem_bits (<= 15) bits, so a slot value and its bucket give the id back
+// exactly.  An empty slot is 0xFFFF, the only value with bit 15 set.  A claim
+// is a plain 16-bit store; lanes of one step that stored into the same slot
+// are sorted out by reading the slot back in visit16_settle (the last store
+// wins, the others move on).
+__device__ __forceinline__ uint32_t zero_half16(uint4 x, uint32_t rr)
+{
+    // bit 15 or 31 set: some 16-bit half of x XOR (rem, rem) is zero (the
+    // borrow of v - 0x00010001 reaches bit 15 of a half only through a zero
+    // half)
+    const uint32_t y0 = x.x ^ rr, y1 = x.y ^ rr, y2 = x.z ^ rr, y3 = x.w ^ rr;
+    return (((y0 - 0x00010001u) & ~y0) | ((y1 - 0x00010001u) & ~y1) | ((y2 - 0x00010001u) & ~y2) |
+            ((y3 - 0x00010001u) & ~y3)) &
+           0x80008000u;
+}
+
+// Bit s of the result: slot s of the eight in x is free (bit 15 of its half).
+__device__ __forceinline__ uint32_t free_mask16(uint4 x)
+{
+    return ((x.x >> 15) & 1u) | ((x.x >> 30) & 2u) | ((x.y >> 13) & 4u) | ((x.y >> 28) & 8u) |
+           ((x.z >> 11) & 16u) | ((x.z >> 26) & 32u) | ((x.w >> 9) & 64u) | ((x.w >> 24) & 128u);
+}
+
+__device__ __forceinline__ bool visit16_begin(uint32_t *tab, uint32_t rem_bits, uint32_t id_mask, bool wide,
+                                              uint32_t id, VisitClaim &c, uint32_t &lossy)
 {
     c.slot = kNone;
     c.old = kNone;
     const uint32_t h = (id * 0x9E3779B1u) & id_mask;
     const uint32_t b = h >> rem_bits;
     const uint32_t rem = h & ((1u << rem_bits) - 1u);
-    const uint4 x = reinterpret_cast<const uint4 *>(tab)[b];
-    // Present: a 16-bit half of some word XOR (rem, rem) is zero (the borrow
-    // of v - 0x00010001 reaches bit 15 of a half only through a zero half).
     const uint32_t rr = rem | (rem << 16);
-    const uint32_t y0 = x.x ^ rr, y1 = x.y ^ rr, y2 = x.z ^ rr, y3 = x.w ^ rr;
-    const uint32_t z = ((y0 - 0x00010001u) & ~y0) | ((y1 - 0x00010001u) & ~y1) | ((y2 - 0x00010001u) & ~y2) |
-                       ((y3 - 0x00010001u) & ~y3);
-    if (z & 0x80008000u)
-        return false;
-    // Slot s is half s & 1 of word s >> 1; free slots have bit 15 set.
-    const uint32_t fm = ((x.x >> 15) & 1u) | ((x.x >> 30) & 2u) | ((x.y >> 13) & 4u) | ((x.y >> 28) & 8u) |
-                        ((x.z >> 11) & 16u) | ((x.z >> 26) & 32u) | ((x.w >> 9) & 64u) | ((x.w >> 24) & 128u);
+    const uint4 *t4 = reinterpret_cast<const uint4 *>(tab);
+    uint32_t fm, nslot;
+    if (wide) {
+        const uint4 x = t4[2u * b], y = t4[2u * b + 1u];
+        if (zero_half16(x, rr) | zero_half16(y, rr))
+            return false;
+        fm = free_mask16(x) | (free_mask16(y) << 8);
+        nslot = 2u * kBucket;
+    } else {
+        const uint4 x = t4[b];
+        if (zero_half16(x, rr))
+            return false;
+        fm = free_mask16(x);
+        nslot = kBucket;
+    }
     if (fm == 0) {
         lossy++;
         return true;
     }
-    const uint32_t s0 = id & (kBucket - 1);
-    const uint32_t f2 = fm | (fm << kBucket);
-    c.slot = b * kBucket + ((s0 + (uint32_t)__builtin_ctz(f2 >> s0)) & (kBucket - 1));
+    // first free slot at or after slot (id mod bucket size), cyclically
+    const uint32_t s0 = id & (nslot - 1u);
+    const uint32_t f2 = fm | (fm << nslot);
+    c.slot = b * nslot + ((s0 + (uint32_t)__builtin_ctz(f2 >> s0)) & (nslot - 1u));
     c.old = rem;
     reinterpret_cast<uint16_t *>(tab)[c.slot] = (uint16_t)rem;
     return true;
 }
 
-__device__ __forceinline__ void visit16_settle(uint32_t *tab, const VisitClaim &c, uint32_t &lossy)
+__device__ __forceinline__ void visit16_settle(uint32_t *tab, bool wide, const VisitClaim &c, uint32_t &lossy)
 {
     if (c.slot == kNone)
         return;
@@ -473,9 +498,10 @@ __device__ __forceinline__ void visit16_settle(uint32_t *tab, const VisitClaim &
     const uint32_t rem = c.old;
     if (t16[c.slot] == rem)
         return;
-    const uint32_t b8 = c.slot & ~(kBucket - 1);
-    for (uint32_t i = 1; i < kBucket; i++) {
-        const uint32_t s = b8 + ((c.slot + i) & (kBucket - 1));
+    const uint32_t nslot = wide ? 2u * kBucket : kBucket;
+    const uint32_t b0 = c.slot & ~(nslot - 1u);
+    for (uint32_t i = 1; i < nslot; i++) {
+        const uint32_t s = b0 + ((c.slot + i) & (nslot - 1u));
         if (t16[s] != 0xFFFFu)
             continue;
         t16[s] = (uint16_t)rem;
@@ -489,8 +515,9 @@ __device__ __forceinline__ bool visit_any(const SearchArgs &a, uint32_t *tab, ui
 {
     if (a.hash16) {
         VisitClaim c;
-        const bool fresh = visit16_begin(tab, a.hash_rem_bits, a.hash_id_mask, id, c, lossy);
-        visit16_settle(tab, c, lossy);
+        const bool wide = a.hash16 == 2u;
+        const bool fresh = visit16_begin(tab, a.hash_rem_bits, a.hash_id_mask, wide, id, c, lossy);
+        visit16_settle(tab, wide, c, lossy);
         return fresh;
     }
     return visit(tab, a.hash_shift + 3u, id, lossy);
@@ -775,6 +802,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
     const uint32_t bshift = a.hash_shift + 3u;  // buckets of eight slots
     const bool h16 = a.hash16 != 0;
+    const bool hwide = a.hash16 == 2u;
     const uint32_t L = st.L;
 
     // k is the pool position of the node to expand and nb already holds its
@@ -894,7 +922,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         // Visited test of the whole row, compaction of the fresh ids.
         VisitClaim claim{kNone, kNone};
         const bool fresh =
-            nb != kNone && (h16 ? visit16_begin(w.hash, a.hash_rem_bits, a.hash_id_mask, nb, claim, st.lossy)
+            nb != kNone && (h16 ? visit16_begin(w.hash, a.hash_rem_bits, a.hash_id_mask, hwide, nb, claim, st.lossy)
                                 : visit_begin(w.hash, bshift, nb, claim, st.lossy));
         const uint64_t fm = ballot(fresh);
         const uint32_t nc = (uint32_t)__popcll(fm);
@@ -955,7 +983,7 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         st.ndc += nc;
         if (fresh) {
             if (h16)
-                visit16_settle(w.hash, claim, st.lossy);
+                visit16_settle(w.hash, hwide, claim, st.lossy);
             else
                 visit_settle(w.hash, nb, claim, st.lossy);
         }

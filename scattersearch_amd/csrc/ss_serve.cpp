@@ -287,7 +287,9 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
         return rc ? rc : SS_ESTATE;
     }
     // Activation shape: LDS per wavefront for k_max/l_max; by default as
-    // many wavefronts as the LDS holds, at most 8 per CU.  Wavefronts stay
+    // many wavefronts as the LDS holds, at most 4 per CU (more concurrent
+    // searches slow each one down: at l_max 512, 7 per CU raised the p99 at
+    // 500k-800k QPS from 1.7 to 2.6-2.9 ms; `blocks` raises it).  Wavefronts stay
     // for linger_us after their last request, so the activation keeps its
     // size under steady load; a second one fills what an activation that
     // thinned out during a quiet spell gave back.
@@ -296,7 +298,7 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
         free_server(s);
         return rc;
     }
-    const uint32_t dflt = std::max(1u, std::min(full, (8u * (uint32_t)s->v.num_cus + wpb - 1) / wpb));
+    const uint32_t dflt = std::max(1u, std::min(full, (4u * (uint32_t)s->v.num_cus + wpb - 1) / wpb));
     s->S = c.ring_slots;
     s->mask = s->S - 1;
     s->ns = c.streams;
@@ -356,11 +358,9 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
         lg++;
     a.search.hash_shift = 32u - lg;
     uint32_t rem_bits = 0, id_mask = 0;
-    a.search.hash16 =
-        ss::use_hash16(s->v.num, s->v.dim_pad, c.l_max, slots, 1, s->v.lds_per_cu,
-                       ss::search_reg_waves_per_cu(s->v.metric, s->v.dim_pad, 1, c.l_max), &rem_bits, &id_mask)
-            ? 1u
-            : 0u;
+    a.search.hash16 = ss::use_hash16(s->v.num, s->v.dim_pad, c.l_max, slots, 1, s->v.lds_per_cu,
+                                     ss::search_reg_waves_per_cu(s->v.metric, s->v.dim_pad, 1, c.l_max), 8,
+                                     &rem_bits, &id_mask);
     a.search.hash_rem_bits = rem_bits;
     a.search.hash_id_mask = id_mask;
     a.search.k_stride = c.k_max;

@@ -25,8 +25,9 @@ def dim_padded(dim):
 
 def _params(mode=_lib.SS_MODE_BATCHED, waves_per_block=0, visited_slots=0,
             blocks=0, use_graph=False, expand_width=0, waves_per_query=0, query_order=0,
-            team_size=0, pool_layout=0, pool_classes=0):
+            team_size=0, pool_layout=0, pool_classes=0, visited_bucket=0):
     p = SearchParams()
+    p.visited_bucket = visited_bucket
     p.query_order = query_order
     p.pool_classes = pool_classes
     p.team_size = team_size
@@ -118,11 +119,13 @@ class Index:
     # -- search -----------------------------------------------------------
     def search(self, queries, K, L, k_stride=None, mode=_lib.SS_MODE_BATCHED,
                waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0,
-               waves_per_query=0, query_order=0, team_size=0, pool_layout=0, pool_classes=0):
+               waves_per_query=0, query_order=0, team_size=0, pool_layout=0, pool_classes=0,
+               visited_bucket=0):
         """Search a batch of host queries.  K and L are ints or per-query
         arrays.  query_order is SS_ORDER_* (scheduling only; results do not
-        depend on it); team_size (4, 8), pool_layout (SS_POOL_*) and
-        pool_classes (SS_CLASSES_*) do not change results either.  Returns (ids [nq][k_stride], dists, stats)."""
+        depend on it); team_size (4, 8), pool_layout (SS_POOL_*),
+        pool_classes (SS_CLASSES_*) and visited_bucket (8, 16) do not change
+        results either.  Returns (ids [nq][k_stride], dists, stats)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q[None, :]
@@ -145,7 +148,7 @@ class Index:
         dists = np.empty((nq, ks), dtype=np.float32)
         stats = np.zeros(nq, dtype=STATS_DTYPE)
         p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query,
-                    query_order, team_size, pool_layout, pool_classes)
+                    query_order, team_size, pool_layout, pool_classes, visited_bucket)
         check(lib.ss_search(self._h, fptr(q), nq, uptr(Ka), uptr(La), k_def, l_def, ks,
                             ctypes.byref(p), uptr(ids), fptr(dists),
                             stats.ctypes.data_as(ctypes.POINTER(QueryStats))))
@@ -169,11 +172,12 @@ class Index:
     def search_device(self, d_queries, nq, k_max, l_max, k_stride, d_out_ids, d_out_dists,
                       d_stats=0, d_K=0, d_L=0, stream=0, mode=_lib.SS_MODE_BATCHED,
                       waves_per_block=0, visited_slots=0, blocks=0, use_graph=False, expand_width=0,
-                      waves_per_query=0, query_order=0, team_size=0, pool_layout=0, pool_classes=0):
+                      waves_per_query=0, query_order=0, team_size=0, pool_layout=0, pool_classes=0,
+                      visited_bucket=0):
         """Enqueue a search on device buffers (raw pointers, e.g. from torch
         tensors' data_ptr()); queries are [nq][dim_pad]."""
         p = _params(mode, waves_per_block, visited_slots, blocks, use_graph, expand_width, waves_per_query,
-                    query_order, team_size, pool_layout, pool_classes)
+                    query_order, team_size, pool_layout, pool_classes, visited_bucket)
         vp = ctypes.c_void_p
         check(lib.ss_search_device(self._h, vp(d_queries), nq, vp(d_K or None), vp(d_L or None),
                                    k_max, l_max, k_stride, ctypes.byref(p), vp(d_out_ids),
@@ -190,7 +194,8 @@ class Index:
                               ctypes.byref(p), ctypes.byref(ms), ctypes.byref(rec)))
         return dict(expand_width=p.expand_width, waves_per_block=p.waves_per_block,
                     visited_slots=p.visited_slots, waves_per_query=p.waves_per_query,
-                    team_size=p.team_size, pool_layout=p.pool_layout, ms=ms.value, recall=rec.value)
+                    team_size=p.team_size, pool_layout=p.pool_layout, visited_bucket=p.visited_bucket,
+                    ms=ms.value, recall=rec.value)
 
     def bruteforce(self, queries, k, exclude=None):
         """Exact top-k by (distance, id) on the GPU."""

@@ -83,11 +83,17 @@ def test_status_codes_match_the_header():
         assert getattr(_lib, name) == defs[name], name
     assert ctypes.sizeof(_lib.QueryStats) == 32 == STATS_DTYPE.itemsize
     assert _lib.QueryStats.status.offset == 12 == STATS_DTYPE.fields["status"][1]
-    assert ctypes.sizeof(_lib.SearchParams) == 44 and _lib.SearchParams.query_order.offset == 28
+    assert ctypes.sizeof(_lib.SearchParams) == 48 and _lib.SearchParams.query_order.offset == 28
     assert _lib.SearchParams.team_size.offset == 32 and _lib.SearchParams.pool_layout.offset == 36
-    assert _lib.SearchParams.pool_classes.offset == 40
-    for f in ("query_order", "team_size", "pool_layout", "pool_classes"):
+    assert _lib.SearchParams.pool_classes.offset == 40 and _lib.SearchParams.visited_bucket.offset == 44
+    for f in ("query_order", "team_size", "pool_layout", "pool_classes", "visited_bucket"):
         assert f"uint32_t {f};" in hdr, f
+    # ss_serve_trace: four doubles, then the two wavefront counts
+    from scattersearch_amd.serve import TRACE_DTYPE
+    assert TRACE_DTYPE.itemsize == 40 and TRACE_DTYPE.fields["waves_alive"][1] == 32
+    for f in ("double submit_us;", "double claim_us;", "double finish_us;", "double seen_us;",
+              "uint32_t waves_alive;", "uint32_t waves_standby;"):
+        assert f in hdr, f
     for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_NOALLOC_MAX_QUERIES", "SS_CLASSES_AUTO",
                  "SS_CLASSES_OFF", "SS_CLASSES_SERIAL"):
         assert getattr(_lib, name) == defs[name], name

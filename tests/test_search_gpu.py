@@ -331,3 +331,64 @@ def test_small_16bit_table_with_lossy_inserts():
     exact = gpu[2]["lossy"] == 0
     assert np.array_equal(gpu[2]["ndc"][exact], ora[2][exact])
     assert (gpu[2]["ndc"] >= ora[2]).all()
+
+
+@pytest.mark.parametrize("slots", [256, 1024, 4096])
+@pytest.mark.parametrize("L", [48, 128, 512])
+def test_wide_16bit_buckets_equal_oracle(c0, slots, L):
+    """The 16-bit visited table in buckets of sixteen slots (two 16-byte reads
+    per lookup): ids, distance bits and hops equal the oracle at every pool
+    layout; 256 slots fill buckets (lossy inserts, recomputed distances
+    only), 4,096 forget nothing up to L = 128, so distance counts match."""
+    import oracle
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"]
+    gpu = ix.search(q, 10, L, visited_slots=slots, visited_bucket=16)
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], q, 10, L)
+    _assert_same(gpu, ora, check_ndc=False, what=f"16-slot buckets, {slots} slots, L={L}")
+    st = gpu[2]
+    exact = st["lossy"] == 0
+    assert np.array_equal(st["ndc"][exact], ora[2][exact])
+    assert (st["ndc"] >= ora[2]).all()
+    if slots == 256 and L >= 128:
+        assert st["lossy"].sum() > 0
+    if slots == 4096 and L <= 128:
+        assert st["lossy"].sum() == 0
+
+
+def test_wide_buckets_forget_less_than_narrow(c0):
+    """At the same table size, buckets of sixteen slots overflow far less
+    often than buckets of eight."""
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"]
+    narrow = ix.search(q, 10, 256, visited_slots=1024, visited_bucket=8)[2]["lossy"].sum()
+    wide = ix.search(q, 10, 256, visited_slots=1024, visited_bucket=16)[2]["lossy"].sum()
+    assert wide < narrow, (wide, narrow)
+
+
+def test_wide_buckets_in_one_launch_mixing_pool_sizes(small):
+    """Per-query K and L through the pool-size classes with 16-slot buckets
+    equal the oracle."""
+    import oracle
+    rng = np.random.default_rng(23)
+    q = small["queries"]
+    nq = q.shape[0]
+    K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(max(int(k), 16), 513) for k in K], dtype=np.uint32)
+    ix = _index(small["base"], small["adj"], small["ep"])
+    gpu = ix.search(q, K, L, k_stride=32, visited_bucket=16, visited_slots=2048)
+    ora = oracle.search_batch(small["base"], small["adj"], small["ep"], q, K, L, k_stride=32)
+    for i in range(nq):
+        k = int(K[i])
+        assert np.array_equal(gpu[0][i, :k], ora[0][i, :k]), f"query {i} K={k} L={L[i]}"
+        assert np.array_equal(gpu[1][i, :k].view(np.uint32), ora[1][i, :k].view(np.uint32))
+        assert gpu[2]["hops"][i] == ora[3][i]
+
+
+def test_visited_bucket_rejects_other_sizes(small):
+    from scattersearch_amd import SSError
+    from scattersearch_amd._lib import SS_EINVAL
+    ix = _index(small["base"], small["adj"], small["ep"])
+    with pytest.raises(SSError) as e:
+        ix.search(small["queries"][:4], 10, 48, visited_bucket=4)
+    assert e.value.code == SS_EINVAL
