@@ -43,7 +43,14 @@ extern "C" {
 #define SS_MODE_BATCHED 0   /* one persistent launch fed from a device queue */
 #define SS_MODE_PER_QUERY 1 /* one launch per query (the plain baseline) */
 
+/* Pools of up to SS_MAX_L entries run the register-merge kernels (and may
+ * use expand_width, waves_per_query, the autotuner and the serving loop);
+ * a launch whose largest L is above it runs the large-pool kernel (pool in
+ * LDS, exact one-wavefront search only), up to SS_MAX_L_LARGE entries as far
+ * as a CU's LDS holds the pool (8 bytes per entry), the query and a
+ * visited table. */
 #define SS_MAX_L 512
+#define SS_MAX_L_LARGE 16384
 #define SS_MAX_K 512
 #define SS_MAX_RPAD 64
 

@@ -23,6 +23,7 @@ SS_QUERY_REJECTED = 0xFFFFFFFF
 SS_QUERY_FAULT = 0x0BAD0001
 SS_NONE = 0xFFFFFFFF
 SS_MAX_L = 512
+SS_MAX_L_LARGE = 16384
 SS_MAX_K = 512
 SS_MAX_RPAD = 64
 

@@ -602,6 +602,45 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
     s->team = team;
     s->pool_lds = layout == SS_POOL_LDS ? 1u : 0u;
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
+    if (l_max > SS_MAX_L) {
+        // The large-pool kernel: one wavefront per workgroup, pool in LDS.
+        // Without a slot count asked for, the table starts at 8 slots per
+        // pool entry and halves until the wavefront fits a CU's LDS (a
+        // smaller table only recomputes distances).
+        if (ew != 1 || wq != 1) {
+            set_error("pools above 512 entries take the exact one-wavefront search (expand_width 1, "
+                      "waves_per_query 1)");
+            return SS_EINVAL;
+        }
+        const bool asked = p && p->visited_slots;
+        if (!asked)
+            slots = std::min<uint32_t>(32768, next_pow2(l_max * 8));
+        size_t per_wave = 0;
+        while (true) {
+            s->hash_rem_bits = s->hash_id_mask = 0;
+            s->hash16 = ss::hash_params16(ix->num, slots, bucket, &s->hash_rem_bits, &s->hash_id_mask)
+                            ? (bucket == 16 ? 2u : 1u)
+                            : 0u;
+            per_wave = ss::large_lds_bytes(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots);
+            if (per_wave <= lds_cu || asked || slots <= 256)
+                break;
+            slots /= 2;
+        }
+        if (per_wave > lds_cu) {
+            set_error("search state does not fit in LDS: the pool takes 8 bytes per entry, lower L or "
+                      "visited_slots");
+            return SS_EINVAL;
+        }
+        const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 16);
+        uint32_t blocks = p && p->blocks ? p->blocks : (uint32_t)ix->num_cus * waves_cu;
+        s->wpb = 1;
+        s->slots = slots;
+        s->blocks = std::max<uint32_t>(1, std::min(blocks, nq));
+        s->lds_wave = per_wave;
+        s->team = 4;
+        s->pool_lds = 1;
+        return SS_OK;
+    }
     if (wq > 1) {  // blocks of wq wavefronts, one query per block
         const size_t per_block = ss::scatter_lds_bytes_per_block(ix->dim_pad, l_max, slots, wq);
         const uint32_t blocks_cu = (uint32_t)std::min<size_t>(lds_cu / per_block, 32 / wq);
@@ -783,8 +822,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
         set_error("search: NULL buffer");
         return SS_EINVAL;
     }
-    if (k_max == 0 || l_max == 0 || k_max > l_max || l_max > SS_MAX_L || k_max > k_stride) {
-        set_error("search: need 1 <= K <= L <= 512 and K <= k_stride");  // tests/search.cpp:54-61
+    if (k_max == 0 || l_max == 0 || k_max > l_max || l_max > SS_MAX_L_LARGE || k_max > k_stride) {
+        set_error("search: need 1 <= K <= L <= 16384 and K <= k_stride");  // tests/search.cpp:54-61
         return SS_EINVAL;
     }
     const uint32_t mode = p ? p->mode : SS_MODE_BATCHED;
@@ -860,7 +899,8 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     // 0..3 count the classes' positions and words 8..15 hold their ranges.
     uint32_t classes = 1;
     const uint32_t cls_mode = p_user ? p_user->pool_classes : SS_CLASSES_AUTO;
-    if (d_L && !d_path && a.waves_per_query == 1 && a.expand_width == 1 && l_max > 64 && cls_mode != SS_CLASSES_OFF)
+    if (d_L && !d_path && a.waves_per_query == 1 && a.expand_width == 1 && l_max > 64 && l_max <= SS_MAX_L &&
+        cls_mode != SS_CLASSES_OFF)
         while (classes < ss::kMaxPoolClasses && (64u << (classes - 1)) < l_max)
             classes++;
     a.classes = classes;
@@ -1081,8 +1121,8 @@ static int search_host(ss_index *ix, const float *queries, uint32_t nq, const ui
         for (uint32_t i = 0; i < nq; i++) {
             const uint32_t ki = K ? K[i] : k_default;
             const uint32_t li = L ? L[i] : l_default;
-            if (ki == 0 || li == 0 || ki > li || li > SS_MAX_L) {
-                set_error("ss_search: query " + std::to_string(i) + " needs 1 <= K <= L <= 512");
+            if (ki == 0 || li == 0 || ki > li || li > SS_MAX_L_LARGE) {
+                set_error("ss_search: query " + std::to_string(i) + " needs 1 <= K <= L <= 16384");
                 return SS_EINVAL;
             }
             k_max = std::max(k_max, ki);

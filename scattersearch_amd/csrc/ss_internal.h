@@ -223,6 +223,11 @@ hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream
 // what: 0 = random initial lists, 1 = one pull iteration
 hipError_t launch_nnd(int metric, const NndArgs &a, int what, uint32_t blocks, hipStream_t stream);
 hipError_t launch_nnd_reverse(const NndArgs &a, hipStream_t stream);
+// Pools above SS_MAX_L (ss_search_large.hip): LDS bytes of one wavefront,
+// the launch (one wavefront per workgroup) and the kernels' LDS attribute.
+size_t large_lds_bytes(uint32_t dim_pad, uint32_t l_max, uint32_t hash_words);
+hipError_t launch_search_large(int metric, const SearchArgs &a, uint32_t blocks, hipStream_t stream);
+hipError_t init_search_large(size_t max_lds);
 // Writes the device wall clock (the counter behind ss_query_stats' times) to
 // device memory at `out`.
 hipError_t launch_read_clock(uint64_t *out, hipStream_t stream);

@@ -1765,6 +1765,8 @@ uint32_t bruteforce_kmax() { return kBfKMax; }
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint32_t wpb,
                          hipStream_t stream)
 {
+    if (a.l_max > SS_MAX_L)
+        return launch_search_large(metric, a, blocks, stream);
     if (a.waves_per_query <= 1 && a.expand_width <= 1 && a.team == 8)
         return launch_search_team8(metric, a, blocks, wpb, stream);
     if (metric == SS_METRIC_INNER_PRODUCT)
@@ -1883,7 +1885,7 @@ size_t nnd_lds_bytes_host(uint32_t dim_pad, uint32_t hash_slots) { return nnd_ld
 hipError_t init_kernel_attributes(size_t max_lds)
 {
     hipError_t e;
-    if ((e = init_search_team8(max_lds)) != hipSuccess)
+    if ((e = init_search_team8(max_lds)) != hipSuccess || (e = init_search_large(max_lds)) != hipSuccess)
         return e;
     if ((e = attr_all<SearchLaunch, SS_METRIC_L2>(max_lds)) != hipSuccess)
         return e;

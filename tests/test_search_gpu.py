@@ -245,7 +245,7 @@ def test_invalid_arguments_rejected(small):
     with pytest.raises(SSError):
         ix.search(small["queries"], 20, 10)  # K > L
     with pytest.raises(SSError):
-        ix.search(small["queries"], 10, 1024)  # L above the pool bound
+        ix.search(small["queries"], 10, 16385)  # L above the pool bound
     with pytest.raises(SSError):
         ix.search(small["queries"], 10, 48, visited_slots=1000)  # not a power of two
     bad = small["adj"].copy()
