@@ -87,6 +87,7 @@ def main():
                     help="skip the wider-expansion and several-wavefronts-per-query measurements")
     ap.add_argument("--visited-slots", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
+    ap.add_argument("--visited-bucket", type=int, default=0, help="16-bit visited-table bucket (8 or 16); 0 = tuned")
     ap.add_argument("--team-size", type=int, default=0, help="lanes per distance (4 or 8); 0 = tuned")
     ap.add_argument("--pool-layout", type=int, default=0, help="SS_POOL_* (1 registers, 2 LDS); 0 = tuned")
     ap.add_argument("--query-order", choices=["auto", "index", "locality"], default="auto",
@@ -99,6 +100,8 @@ def main():
                     help="Poisson arrival rates (QPS) of the serving sweep; empty to skip")
     ap.add_argument("--batch-sizes", default="1,64,1024,10000",
                     help="also time hipGraph-replayed launches of these batch sizes; empty to skip")
+    ap.add_argument("--large-L", default="1024,1600",
+                    help="also time batched launches at these pools above 512 entries; empty to skip")
     ap.add_argument("--serve-config", default="",
                     help='serving knobs as JSON, e.g. {"standby_waves": 1024} (ss_serve_config fields)')
     ap.add_argument("--serve-seconds", type=float, default=0.5,
@@ -117,6 +120,8 @@ def main():
         if "--batch-sizes" not in sys.argv:
             args.batch_sizes = "1,64,1024,10000,100000"
         args.no_variants = True
+        if "--large-L" not in sys.argv:
+            args.large_L = ""
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the query path "
@@ -181,7 +186,8 @@ def main():
                          d_stats.data_ptr(), stream=stream.cuda_stream, use_graph=use_graph,
                          visited_slots=slots, waves_per_block=wpb, expand_width=ew, waves_per_query=wq,
                          query_order=qorder, team_size=team if ew in (0, 1) and wq in (0, 1) else 0,
-                         pool_layout=layout if ew in (0, 1) and wq in (0, 1) else 0)
+                         pool_layout=layout if ew in (0, 1) and wq in (0, 1) else 0,
+                         visited_bucket=args.visited_bucket if ew in (0, 1) and wq in (0, 1) else 0)
 
     def stats_host():
         st = d_stats.cpu().numpy().view(np.uint32).reshape(-1).view(ss.STATS_DTYPE)
@@ -287,6 +293,15 @@ def main():
         f"recall {rec:.4f}; distances recomputed after the visited hash forgot an id: "
         f"{float((st['ndc'] - ndc_exact).mean()):.1f} per query (not counted)")
 
+    # The same batch through host buffers (ss_search: queries copied in and
+    # results copied back over PCIe around the launch), for reference.
+    ix.search(queries, K, L)
+    t_h = time.perf_counter()
+    for _ in range(3):
+        ix.search(queries, K, L)
+    host_qps = nq * 3 / (time.perf_counter() - t_h)
+    log(f"host-buffer batches (PCIe copies included): {host_qps:.0f} QPS")
+
     # Plain kernel: one launch per query.
     npq = min(args.plain_queries, nq)
     from scattersearch_amd import SS_MODE_PER_QUERY
@@ -354,6 +369,18 @@ def main():
             variants.append(dict(expand_width=ew, waves_per_query=wq, best_at_target=best_v, sweep=rows))
             log(f"variant ew={ew} wq={wq}: best at recall>={args.recall_target}: {best_v}")
 
+    # Pools above 512 entries (the large-pool kernel; the reference's own
+    # sweeps run L up to 1,600 and 8,000-16,000).
+    large = []
+    for Lg in [int(x) for x in args.large_L.split(",") if x]:
+        wall_g, kern_g = timed(Lg, 2, 1)
+        st_g = stats_host()
+        large.append(dict(L=Lg, recall=round(recall_of(d_ids.cpu().numpy().view(np.uint32)), 4),
+                          qps=round(nq * 2 / wall_g, 1), kernel_ms=round(kern_g, 3),
+                          mean_ndc=round(float(st_g["ndc"].mean()), 1),
+                          mean_hops=round(float(st_g["hops"].mean()), 1)))
+        log(f"large pool {large[-1]}")
+
     serving = None
     if args.serve_rates:
         try:
@@ -387,10 +414,12 @@ def main():
                    "launch": dict({k: tuned[k] for k in ("expand_width", "waves_per_block", "visited_slots",
                                                          "visited_bucket", "team_size", "pool_layout")},
                                   query_order=order_used(nq))},
+        "host_buffers_qps": round(host_qps, 1),
         "plain_qps": round(plain_qps, 1),
         "speedup_vs_plain": round(qps / plain_qps, 2),
         "serving": serving,
         "variants": variants,
+        "large_pools": large,
         "batches": batches,
         "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
                        "service_p99": round(metrics.percentile(service_ms, 0.99), 4),
@@ -549,22 +578,22 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
 
-    def launch(classes):
+    def launch(classes, slots=args.visited_slots, bucket=0):
         ix.search_device(dq.data_ptr(), nq, k_max, l_max, k_max, d_ids.data_ptr(), d_dist.data_ptr(),
                          d_stats.data_ptr(), d_K=dK.data_ptr(), d_L=dL.data_ptr(), stream=stream.cuda_stream,
-                         use_graph=True, visited_slots=args.visited_slots, waves_per_block=args.waves_per_block,
-                         pool_classes=classes)
+                         use_graph=True, visited_slots=slots, waves_per_block=args.waves_per_block,
+                         pool_classes=classes, visited_bucket=bucket)
 
-    def timed(classes):
+    def timed(classes, slots=args.visited_slots, bucket=0):
         for _ in range(args.warmup):
-            launch(classes)
+            launch(classes, slots, bucket)
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(args.steps)]
         t_wall = time.perf_counter()
         for e0, e1 in evs:
             e0.record(stream)
-            launch(classes)
+            launch(classes, slots, bucket)
             e1.record(stream)
         torch.cuda.synchronize()
         wall_ = time.perf_counter() - t_wall
@@ -578,7 +607,7 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         from scattersearch_amd._lib import SS_CLASSES_AUTO
         with torch.cuda.stream(stream):
             for _ in range(args.steps):
-                launch(SS_CLASSES_AUTO)
+                launch(SS_CLASSES_AUTO, args.visited_slots, args.visited_bucket)
             src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
             dst = torch.empty_like(src)
             for _ in range(5):
@@ -590,18 +619,31 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
     # one kernel for every pool size, the class kernels one after another,
     # then side by side (the default, whose figures are reported); all three
     # return the same results
+    # The concurrent classes also run with one visited table for every class
+    # (slots, slots per bucket) beside the per-class tuned ones: the classes
+    # were tuned one at a time, and side by side they share the CUs; the
+    # fastest is the reported launch.  Every variant returns the same results.
     from scattersearch_amd._lib import SS_CLASSES_AUTO, SS_CLASSES_OFF, SS_CLASSES_SERIAL
     variants = {}
     ref_ids = None
-    for name, mode in (("one_kernel", SS_CLASSES_OFF), ("classes_serial", SS_CLASSES_SERIAL),
-                       ("classes_concurrent", SS_CLASSES_AUTO)):
-        wall, kern_ms, ids, st = timed(mode)
-        variants[name] = round(kern_ms, 4)
-        log(f"mixed launch, {name}: {kern_ms:.3f} ms per launch")
+    best = None
+    tables = [(args.visited_slots, args.visited_bucket)]
+    if not args.visited_slots:
+        tables += [(4096, 8), (4096, 16), (2048, 16)]
+    runs = [("one_kernel", SS_CLASSES_OFF, tables[0]), ("classes_serial", SS_CLASSES_SERIAL, tables[0])]
+    runs += [("classes_concurrent" + ("" if tb == tables[0] else f"_slots{tb[0]}_bucket{tb[1]}"), SS_CLASSES_AUTO, tb)
+             for tb in tables]
+    for name, mode, (slots_v, bucket_v) in runs:
+        wall_v, kern_v, ids, st = timed(mode, slots_v, bucket_v)
+        variants[name] = round(kern_v, 4)
+        log(f"mixed launch, {name}: {kern_v:.3f} ms per launch")
         if ref_ids is None:
             ref_ids, ref_hops = ids.copy(), st["hops"].copy()
         elif not np.array_equal(ids, ref_ids) or not np.array_equal(st["hops"], ref_hops):
             raise RuntimeError(f"the mixed launch ({name}) returned other results than one kernel")
+        if mode == SS_CLASSES_AUTO and (best is None or kern_v < best[2]):
+            best = (name, wall_v, kern_v, ids, st, slots_v, bucket_v)
+    chosen, wall, kern_ms, ids, st, slots_c, bucket_c = best
     rec = {}
     for k in (1, 10, 100):
         sel = Ks == k
@@ -630,7 +672,9 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         "config": {"workload": w.name, "num": ix.info()["num"], "dim": w.dim, "nq": nq, "R": w.R,
                    "metric": "L2" if w.metric == 0 else "inner_product",
                    "K_mix": {str(k): int((Ks == k).sum()) for k in (1, 10, 100)}, "L_mean": float(Ls.mean()),
-                   "launch": {"pool_classes": "concurrent",
+                   "launch": {"pool_classes": "concurrent", "variant": chosen,
+                              "visited_slots": slots_c or "tuned per class",
+                              "visited_bucket": bucket_c or "tuned per class",
                               "classes": {f"L<={b}": {k: t[k] for k in ("waves_per_block", "visited_slots",
                                                                           "visited_bucket", "team_size",
                                                                           "pool_layout")}

@@ -313,7 +313,9 @@ typedef struct ss_serve_config {
     uint32_t ring_slots;      /* most requests in flight (power of two); 0 = 16384 */
     uint32_t visited_slots;   /* per-query visited hash; 0 = tuned or default */
     uint32_t waves_per_block; /* 0 = tuned or default */
-    uint32_t blocks;          /* blocks per activation; 0 = what the LDS holds, <= 4 waves per CU */
+    uint32_t blocks;          /* blocks per activation (times the waves per block: the
+                                 server's wavefront budget, which activations running
+                                 together share); 0 = what the LDS holds, <= 4 waves per CU */
     uint32_t streams;         /* activations alive at once, 1..8; 0 = 2 */
     uint32_t linger_us;       /* how long a standby wavefront waits for work, <= 200000; 0 = 20000 */
     uint32_t admit_us;        /* an unclaimed request this old starts another activation; 0 = 50 */

@@ -118,7 +118,7 @@ struct ServeArgs {
     uint32_t mask;          // ring slots - 1
     uint32_t max_idle_polls;
     uint32_t max_standby;   // idle wavefronts kept per server; others leave
-    uint32_t pad;
+    uint32_t max_alive;     // wavefronts of all activations together; a later one only refills
     uint64_t linger_ticks;  // device wall-clock ticks a standby wavefront waits for work
     uint64_t refresh_ticks; // least wall-clock ticks between reads of host memory
 };

@@ -800,8 +800,17 @@ __global__ __launch_bounds__(512, search_min_waves<NBLK>()) void serve_kernel(Se
     bool standby = false;  // counted in ctl.standby
     uint64_t idle_since = 0;
     uint32_t polls = 0, sleeps = 1;
+    // The server's wavefront budget holds across activations: a wavefront of
+    // an activation started while an earlier one still runs only fills what
+    // that one has given back.
+    uint32_t before = 0;
     if (lane == 0)
-        __hip_atomic_fetch_add(&a.ctl->alive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        before = __hip_atomic_fetch_add(&a.ctl->alive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (uniform(before) >= a.max_alive) {
+        if (lane == 0)
+            __hip_atomic_fetch_add(&a.ctl->alive, 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     while (true) {
         uint64_t tk = kNoTicket;
         if (lane == 0) {

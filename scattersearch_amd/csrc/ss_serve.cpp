@@ -289,7 +289,9 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     // Activation shape: LDS per wavefront for k_max/l_max; by default as
     // many wavefronts as the LDS holds, at most 4 per CU (more concurrent
     // searches slow each one down: at l_max 512, 7 per CU raised the p99 at
-    // 500k-800k QPS from 1.7 to 2.6-2.9 ms; `blocks` raises it).  Wavefronts stay
+    // 500k-800k QPS from 1.7 to 2.6-2.9 ms; `blocks` raises it).  One
+    // activation's size is the server's budget: activations running together
+    // never hold more wavefronts than that (serve_kernel).  Wavefronts stay
     // for linger_us after their last request, so the activation keeps its
     // size under steady load; a second one fills what an activation that
     // thinned out during a quiet spell gave back.
@@ -377,6 +379,7 @@ int ss_serve_start(ss_index *ix, const ss_serve_config *config, ss_server **out)
     a.mask = s->mask;
     a.max_idle_polls = kMaxIdlePolls;
     a.max_standby = c.standby_waves;
+    a.max_alive = s->blocks * s->wpb;
     uint32_t khz = 0;
     if ((rc = ss_wall_clock_khz(ix, &khz))) {
         free_server(s);
