@@ -180,9 +180,13 @@ static void dispatch_loop(ss_server *s)
         // a later claim of the same slot implies this one (claims are in order)
         while (scan < pub && load_acquire(s->h_claimed + (scan & s->mask)) >= scan + 1)
             scan++;
+        // Another activation only while the running ones hold less than the
+        // server's budget (as the GPU last reported it): its wavefronts would
+        // otherwise leave at once.
+        const bool room = __atomic_load_n(&s->h_ctl->mirror[0], __ATOMIC_RELAXED) < s->args.max_alive;
         bool launched = false;
         if (scan < pub && alive < s->ns &&
-            (alive == 0 || clk::now() - s->t_pub[scan & s->mask] >= admit)) {
+            (alive == 0 || (room && clk::now() - s->t_pub[scan & s->mask] >= admit))) {
             for (uint32_t k = 0; k < s->ns; k++) {
                 const uint32_t i = (next + k) % s->ns;
                 if (s->busy[i])
