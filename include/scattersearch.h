@@ -104,7 +104,11 @@ typedef struct ss_search_params {
                                  entries are read into registers once per
                                  step; SS_POOL_LDS (2): the pool is scanned
                                  in LDS and read again only for steps that
-                                 admit a candidate (the only layout above 64) */
+                                 admit a candidate; SS_POOL_SHIFT (3): the pool
+                                 in LDS, each admitted candidate placed by a
+                                 binary search and only the entries after the
+                                 first placed one moved (the layout of every
+                                 launch whose L is above SS_MAX_L) */
     uint32_t pool_classes;    /* launches with per-query L above 64
                                  (scheduling only, results do not depend on
                                  it): SS_CLASSES_AUTO (0): the queries are
@@ -132,6 +136,7 @@ typedef struct ss_search_params {
 #define SS_POOL_AUTO 0u
 #define SS_POOL_REGISTERS 1u
 #define SS_POOL_LDS 2u
+#define SS_POOL_SHIFT 3u
 
 #define SS_ORDER_AUTO 0u
 #define SS_ORDER_INDEX 1u

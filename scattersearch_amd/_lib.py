@@ -39,6 +39,7 @@ SS_NOALLOC_MAX_QUERIES = 131072
 SS_POOL_AUTO = 0
 SS_POOL_REGISTERS = 1
 SS_POOL_LDS = 2
+SS_POOL_SHIFT = 3
 
 SS_CLASSES_AUTO = 0
 SS_CLASSES_OFF = 1

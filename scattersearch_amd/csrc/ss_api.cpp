@@ -595,25 +595,26 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_EINVAL;
     }
     const uint32_t layout = p ? p->pool_layout : SS_POOL_AUTO;
-    if (layout > SS_POOL_LDS || (layout == SS_POOL_REGISTERS && l_max > 64)) {
-        set_error("pool_layout must be SS_POOL_AUTO, SS_POOL_REGISTERS (L <= 64) or SS_POOL_LDS");
+    if (layout > SS_POOL_SHIFT || (layout == SS_POOL_REGISTERS && l_max > 64)) {
+        set_error("pool_layout must be SS_POOL_AUTO, SS_POOL_REGISTERS (L <= 64), SS_POOL_LDS or SS_POOL_SHIFT");
         return SS_EINVAL;
     }
     s->team = team;
     s->pool_lds = layout == SS_POOL_LDS ? 1u : 0u;
     const size_t lds_cu = ix->lds_per_cu ? ix->lds_per_cu : 65536;
-    if (l_max > SS_MAX_L) {
-        // The large-pool kernel: one wavefront per workgroup, pool in LDS.
-        // Without a slot count asked for, the table starts at 8 slots per
-        // pool entry and halves until the wavefront fits a CU's LDS (a
-        // smaller table only recomputes distances).
+    if (l_max > SS_MAX_L || layout == SS_POOL_SHIFT) {
+        // The shift-merge kernel (the only one above SS_MAX_L): one wavefront
+        // per workgroup, pool in LDS.  Without a slot count asked for, above
+        // SS_MAX_L the table starts at 8 slots per pool entry and halves until
+        // the wavefront fits a CU's LDS (a smaller table only recomputes
+        // distances).
         if (ew != 1 || wq != 1) {
-            set_error("pools above 512 entries take the exact one-wavefront search (expand_width 1, "
-                      "waves_per_query 1)");
+            set_error("the shift-merge pool layout (every pool above 512 entries) takes the exact "
+                      "one-wavefront search (expand_width 1, waves_per_query 1)");
             return SS_EINVAL;
         }
         const bool asked = p && p->visited_slots;
-        if (!asked)
+        if (!asked && l_max > SS_MAX_L)
             slots = std::min<uint32_t>(32768, next_pow2(l_max * 8));
         size_t per_wave = 0;
         while (true) {
@@ -638,7 +639,7 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         s->blocks = std::max<uint32_t>(1, std::min(blocks, nq));
         s->lds_wave = per_wave;
         s->team = 4;
-        s->pool_lds = 1;
+        s->pool_lds = 2;  // the shift-merge kernel
         return SS_OK;
     }
     if (wq > 1) {  // blocks of wq wavefronts, one query per block
@@ -1701,8 +1702,9 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
     if (b1 && b1->p.expand_width == 1 && b1->p.waves_per_query == 1) {
         const ss_search_params base_p = b1->p;
         for (uint32_t team : {4u, 8u})
-            for (uint32_t layout : {SS_POOL_REGISTERS, SS_POOL_LDS}) {
-                if ((layout == SS_POOL_REGISTERS && L > 64) || (team == 4 && layout == dflt_layout))
+            for (uint32_t layout : {SS_POOL_REGISTERS, SS_POOL_LDS, SS_POOL_SHIFT}) {
+                if ((layout == SS_POOL_REGISTERS && L > 64) || (team == 4 && layout == dflt_layout) ||
+                    (team == 8 && layout == SS_POOL_SHIFT))  // the shift-merge kernel has teams of 4
                     continue;
                 ss_search_params p = base_p;
                 p.team_size = team;

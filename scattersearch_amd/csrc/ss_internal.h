@@ -46,7 +46,8 @@ struct SearchArgs {
     uint32_t num;          // base points (random entry points of the multi-wavefront search)
     const uint32_t *order; // non-null: the work queue's position i holds query order[i]
     uint32_t team;         // lanes per distance (4 or 8) of the one-wavefront search
-    uint32_t pool_lds;     // 1: pools of at most 64 entries take the LDS-scanned body
+    uint32_t pool_lds;     // 1: pools of at most 64 entries take the LDS-scanned body; 2: the
+                           // shift-merge kernel (ss_search_large.hip), any pool size
     const uint32_t *range; // non-null: this launch takes work-queue positions range[0] .. +range[1]
     uint32_t classes;      // pool-size classes of the launch this kernel is part of (cache key)
 };

@@ -1774,7 +1774,7 @@ uint32_t bruteforce_kmax() { return kBfKMax; }
 hipError_t launch_search(int metric, const SearchArgs &a, uint32_t blocks, uint32_t wpb,
                          hipStream_t stream)
 {
-    if (a.l_max > SS_MAX_L)
+    if (a.l_max > SS_MAX_L || a.pool_lds == 2)
         return launch_search_large(metric, a, blocks, stream);
     if (a.waves_per_query <= 1 && a.expand_width <= 1 && a.team == 8)
         return launch_search_team8(metric, a, blocks, wpb, stream);

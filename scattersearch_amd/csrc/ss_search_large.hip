@@ -21,7 +21,8 @@
 // table, team distances, tail filter and output), so results are the
 // reference's: ids, distance bits, distance counts and hops equal the CPU
 // restatement's.  One wavefront per workgroup; the LDS of a wavefront is
-// carved for the launch's largest L.
+// carved for the launch's largest L.  Launches with smaller pools may take
+// this kernel too (pool layout SS_POOL_SHIFT), as may a pool-size class.
 #include "ss_search_device.h"
 
 namespace ss {
@@ -239,6 +240,15 @@ __global__ __launch_bounds__(64) void search_large_kernel(SearchArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const LargeLds s = carve_large(smem, a.dim_pad, a.l_max);
+    // A pool-size class of a launch takes positions range[0] .. range[0] +
+    // range[1] of the order.
+    uint32_t rbeg = 0, rcnt = a.nq;
+    if (a.range) {
+        rbeg = uniform(a.range[0]);
+        rcnt = uniform(a.range[1]);
+        if (rbeg > a.nq || rcnt > a.nq - rbeg)
+            rcnt = 0;
+    }
     bool first = true;
     while (true) {
         uint32_t q;
@@ -247,9 +257,9 @@ __global__ __launch_bounds__(64) void search_large_kernel(SearchArgs a)
             if (lane_id() == 0)
                 pos = atomicAdd(a.queue, 1u);
             pos = uniform(pos);
-            if (pos >= a.nq)
+            if (pos >= rcnt)
                 break;
-            q = a.order ? uniform(a.order[pos]) : pos;
+            q = a.order ? uniform(a.order[rbeg + pos]) : rbeg + pos;
             if (q >= a.nq)  // a malformed order entry: skip the position
                 continue;
         } else {

@@ -94,7 +94,8 @@ def test_status_codes_match_the_header():
     for f in ("double submit_us;", "double claim_us;", "double finish_us;", "double seen_us;",
               "uint32_t waves_alive;", "uint32_t waves_standby;"):
         assert f in hdr, f
-    for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_NOALLOC_MAX_QUERIES", "SS_CLASSES_AUTO",
+    for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_POOL_SHIFT", "SS_NOALLOC_MAX_QUERIES",
+                 "SS_CLASSES_AUTO",
                  "SS_CLASSES_OFF", "SS_CLASSES_SERIAL"):
         assert getattr(_lib, name) == defs[name], name
 

@@ -173,3 +173,36 @@ def test_large_pool_limits(small):
         with pytest.raises(SSError) as e:
             ix.search(q, 10, L, **kw)
         assert e.value.code == SS_EINVAL, (L, kw)
+
+
+@pytest.mark.parametrize("L", [16, 48, 64, 200, 512])
+def test_shift_layout_at_small_pools_equals_oracle(c0, L):
+    """The shift-merge kernel asked for at pools of 512 entries or fewer
+    (pool_layout SS_POOL_SHIFT) gives the oracle's results, like the kernels
+    it replaces there."""
+    import oracle
+    from scattersearch_amd._lib import SS_POOL_SHIFT
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"][:64]
+    gpu = ix.search(q, 10, L, pool_layout=SS_POOL_SHIFT)
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], q, 10, L)
+    _assert_same(gpu, ora, f"shift layout L={L}")
+    ref = ix.search(q, 10, L)
+    assert np.array_equal(gpu[0], ref[0]) and np.array_equal(gpu[2]["hops"], ref[2]["hops"])
+
+
+def test_shift_layout_in_pool_size_classes(small):
+    """Per-query L split into pool-size classes, every class on the
+    shift-merge kernel (each takes only its own range of the order)."""
+    import oracle
+    from scattersearch_amd._lib import SS_POOL_SHIFT, SS_CLASSES_AUTO, SS_CLASSES_SERIAL
+    rng = np.random.default_rng(41)
+    q = small["queries"]
+    nq = q.shape[0]
+    K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(max(int(k), 16), 513) for k in K], dtype=np.uint32)
+    ix = _index(small["base"], small["adj"], small["ep"])
+    ora = oracle.search_batch(small["base"], small["adj"], small["ep"], q, K, L, k_stride=32)
+    for classes in (SS_CLASSES_AUTO, SS_CLASSES_SERIAL):
+        gpu = ix.search(q, K, L, k_stride=32, pool_layout=SS_POOL_SHIFT, pool_classes=classes, use_graph=True)
+        _assert_same(gpu, ora, f"shift layout, classes {classes}", K=K)

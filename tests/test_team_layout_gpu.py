@@ -91,7 +91,7 @@ def test_bad_team_and_layout_refused(small):
     ix = _index(small["base"], small["adj"], small["ep"])
     q = small["queries"][:4]
     for kw in ({"team_size": 2}, {"team_size": 16}, {"team_size": 8, "expand_width": 2},
-               {"team_size": 8, "waves_per_query": 2}, {"pool_layout": 3}):
+               {"team_size": 8, "waves_per_query": 2}, {"pool_layout": 4}):
         with pytest.raises(SSError) as e:
             ix.search(q, 10, 48, **kw)
         assert e.value.code == SS_EINVAL, kw
