@@ -605,28 +605,50 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
     if (l_max > SS_MAX_L || layout == SS_POOL_SHIFT) {
         // The shift-merge kernel (the only one above SS_MAX_L): one wavefront
         // per workgroup, pool in LDS.  Without a slot count asked for, above
-        // SS_MAX_L the table starts at 8 slots per pool entry and halves until
-        // the wavefront fits a CU's LDS (a smaller table only recomputes
-        // distances).
+        // SS_MAX_L the table is the largest from 2,048 slots up that keeps as
+        // many wavefronts per CU as 2,048 slots do: the pool already takes
+        // 8 bytes per entry, and a forgotten id only costs a recomputed
+        // distance, which at these pools is cheaper than a wavefront fewer per
+        // CU (1M x 128, L = 1,600: 2,048 slots 25 ms per 10k queries against
+        // 48 ms with 16,384).  16-bit slots in buckets of 16 unless asked
+        // otherwise.
         if (ew != 1 || wq != 1) {
             set_error("the shift-merge pool layout (every pool above 512 entries) takes the exact "
                       "one-wavefront search (expand_width 1, waves_per_query 1)");
             return SS_EINVAL;
         }
         const bool asked = p && p->visited_slots;
-        if (!asked && l_max > SS_MAX_L)
-            slots = std::min<uint32_t>(32768, next_pow2(l_max * 8));
-        size_t per_wave = 0;
-        while (true) {
-            s->hash_rem_bits = s->hash_id_mask = 0;
-            s->hash16 = ss::hash_params16(ix->num, slots, bucket, &s->hash_rem_bits, &s->hash_id_mask)
-                            ? (bucket == 16 ? 2u : 1u)
-                            : 0u;
-            per_wave = ss::large_lds_bytes(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots);
-            if (per_wave <= lds_cu || asked || slots <= 256)
-                break;
-            slots /= 2;
+        const uint32_t lbucket = (p && p->visited_bucket) ? bucket : (l_max > SS_MAX_L ? 16u : bucket);
+        auto wave_bytes = [&](uint32_t n, uint32_t *mode, uint32_t *rem, uint32_t *mask) {
+            *rem = *mask = 0;
+            *mode = ss::hash_params16(ix->num, n, lbucket, rem, mask) ? (lbucket == 16 ? 2u : 1u) : 0u;
+            return ss::large_lds_bytes(ix->dim_pad, l_max, *mode ? n / 2 : n);
+        };
+        uint32_t mode = 0, rem = 0, mask = 0;
+        size_t per_wave = wave_bytes(slots, &mode, &rem, &mask);
+        if (!asked && l_max > SS_MAX_L) {
+            slots = 2048;
+            per_wave = wave_bytes(slots, &mode, &rem, &mask);
+            const size_t w0 = per_wave <= lds_cu ? lds_cu / per_wave : 0;
+            while (w0 > 0 && slots < 32768) {
+                uint32_t m2 = 0, r2 = 0, k2 = 0;
+                const size_t b2 = wave_bytes(slots * 2, &m2, &r2, &k2);
+                if (b2 > lds_cu || std::min<size_t>(lds_cu / b2, 16) < std::min<size_t>(w0, 16))
+                    break;
+                slots *= 2;
+                per_wave = b2;
+                mode = m2;
+                rem = r2;
+                mask = k2;
+            }
+            while (per_wave > lds_cu && slots > 256) {  // a pool that leaves no room for 2,048 slots
+                slots /= 2;
+                per_wave = wave_bytes(slots, &mode, &rem, &mask);
+            }
         }
+        s->hash16 = mode;
+        s->hash_rem_bits = rem;
+        s->hash_id_mask = mask;
         if (per_wave > lds_cu) {
             set_error("search state does not fit in LDS: the pool takes 8 bytes per entry, lower L or "
                       "visited_slots");
