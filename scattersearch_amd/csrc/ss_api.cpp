@@ -928,22 +928,27 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
             classes++;
     a.classes = classes;
     // A launch split by pool size takes each class longest pool first unless
-    // the locality order is asked for by name.
-    const bool locality = !d_path && a.waves_per_query == 1 &&
+    // the locality order is asked for by name; so does one kernel over
+    // per-query pools above 64 entries (a long search started last would
+    // end the launch alone).
+    const bool by_length = d_L && !d_path && a.waves_per_query == 1 && classes == 1 && l_max > 64 &&
+                           p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH;
+    const bool locality = !d_path && a.waves_per_query == 1 && !by_length &&
                           (p->query_order == SS_ORDER_LOCALITY ||
                            (p->query_order == SS_ORDER_AUTO && nq >= SS_ORDER_MIN_BATCH && classes == 1));
     ss::OrderArgs oa;
     std::memset(&oa, 0, sizeof(oa));
-    if (locality || classes > 1) {
+    if (locality || classes > 1 || by_length) {
         oa.queries = d_queries;
         oa.nq = nq;
         oa.dim_pad = ix->dim_pad;
         oa.queue = a.queue;
         oa.queues = a.queue;
         oa.classes = classes;
-        oa.Ls = classes > 1 ? d_L : nullptr;
+        oa.Ls = classes > 1 || by_length ? d_L : nullptr;
         oa.ranges = classes > 1 ? a.queue + 8 : nullptr;
-        oa.longest_first = classes > 1 && p->query_order == SS_ORDER_AUTO ? 1u : 0u;
+        oa.longest_first = (classes > 1 && p->query_order == SS_ORDER_AUTO) || by_length ? 1u : 0u;
+        oa.l_top = l_max;
         if (scratch) {
             // pivots of this index, made again each call: every index on the
             // device shares the array
@@ -1008,7 +1013,7 @@ static int run_search(ss_index *ix, const float *d_queries, uint32_t nq, const u
     } parts{parts_buf, nparts};
     const bool serial = cls_mode == SS_CLASSES_SERIAL || parts.size() == 1;
     auto enqueue = [&]() -> hipError_t {
-        hipError_t x = (locality || classes > 1) ? ss::launch_order(oa, (uint32_t)ix->num_cus, st)
+        hipError_t x = (locality || classes > 1 || by_length) ? ss::launch_order(oa, (uint32_t)ix->num_cus, st)
                                                  : ss::launch_queue_reset(a.queue, st);
         if (x != hipSuccess)
             return x;

@@ -89,6 +89,7 @@ struct OrderArgs {
     uint32_t *queues;      // [classes] counters cleared here (queues[0] == queue)
     uint32_t *ranges;      // [2 * classes] out: first sorted position and count of each class
     uint32_t longest_first;  // key_dims == 0: order each class by L, longest first
+    uint32_t l_top;          // one class with longest_first: the pool sizes span 1..l_top
 };
 
 // A serving activation: wavefronts claim published requests from a ring in

@@ -51,9 +51,10 @@ __global__ __launch_bounds__(256) void order_pivot_kernel(const float *base, uin
 // is keyed.  Key = the nearest pivot in squared distance (lowest index on
 // ties), plus kOrderPivots times the query's pool-size class when the launch
 // has several; with no pivots (key_dims 0) and longest_first set, the place
-// of the query's L in its class, longest first, takes the pivot's place, so that each class kernel
-// starts its longest searches first and ends on short ones.  Block 0 also
-// clears the classes' work-queue counters.
+// of the query's L in its class (in 1..l_top when there is one class),
+// longest first, takes the pivot's place, so that each kernel starts its
+// longest searches first and ends on short ones.  Block 0 also clears the
+// classes' work-queue counters.
 constexpr uint32_t kOrderKeyWaves = 4;
 
 __device__ __forceinline__ uint32_t pool_class(uint32_t L, uint32_t classes)
@@ -64,12 +65,13 @@ __device__ __forceinline__ uint32_t pool_class(uint32_t L, uint32_t classes)
     return c;
 }
 
-// Place of L among the pool sizes of its class, longest first: 0 for the
-// largest L the class holds (and for L beyond it, which the search rejects
-// at once), kOrderPivots - 1 for the smallest.
-__device__ __forceinline__ uint32_t longest_first(uint32_t L, uint32_t c)
+// Place of L among the pool sizes lo..hi of its class, longest first: 0 for
+// the largest L the class holds (and for L beyond it, which the search
+// rejects at once), kOrderPivots - 1 for the smallest.
+__device__ __forceinline__ uint32_t longest_first(uint32_t L, uint32_t c, uint32_t classes, uint32_t l_top)
 {
-    const uint32_t hi = 64u << c, lo = c ? (32u << c) + 1u : 1u;
+    const uint32_t hi = classes > 1 ? 64u << c : max(l_top, 1u);
+    const uint32_t lo = classes > 1 && c ? (32u << c) + 1u : 1u;
     if (L >= hi)
         return 0u;
     if (L < lo)
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(64 * kOrderKeyWaves) void order_key_kernel(OrderArg
             if (lane == 0) {
                 const uint32_t L = a.Ls ? a.Ls[q] : 0u;
                 const uint32_t c = a.Ls ? pool_class(L, a.classes) : 0u;
-                a.keys[q] = c * kOrderPivots + (a.Ls && a.longest_first ? longest_first(L, c) : 0u);
+                a.keys[q] = c * kOrderPivots + (a.Ls && a.longest_first ? longest_first(L, c, a.classes, a.l_top) : 0u);
             }
         return;
     }
@@ -1811,7 +1813,7 @@ hipError_t launch_order(const OrderArgs &a, uint32_t num_cus, hipStream_t stream
 {
     if (a.key_dims > kOrderKeyDims || a.key_dims > a.dim_pad || (a.key_dims & 1u) || a.nq == 0 || a.classes == 0 ||
         a.classes > kMaxPoolClasses || !a.queues || (a.classes > 1 && (!a.Ls || !a.ranges)) ||
-        (a.key_dims && !a.pivots))
+        (a.key_dims && !a.pivots) || (a.longest_first && !a.Ls))
         return hipErrorInvalidValue;
     const uint32_t per_block = 2 * kOrderKeyWaves;  // queries per workgroup and pass
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((a.nq + per_block - 1) / per_block,

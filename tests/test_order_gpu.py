@@ -57,6 +57,33 @@ def test_locality_order_with_per_query_k_and_l(big):
     _same(a, b, "per-query K and L")
 
 
+def test_longest_first_order_of_one_kernel(big):
+    """A batch of at least SS_ORDER_MIN_BATCH queries with per-query L above
+    64 searched by one kernel (pool-size classes off, and the large-pool
+    kernel) takes its queries longest pool first: same results as the index
+    order and, on a sample, as the oracle."""
+    import oracle
+    from scattersearch_amd._lib import SS_CLASSES_OFF, SS_ORDER_INDEX, SS_ORDER_MIN_BATCH
+    rng = np.random.default_rng(9)
+    nq = SS_ORDER_MIN_BATCH + 700
+    q = big["queries"][:nq]
+    for lo, hi in ((1, 257), (400, 700)):
+        K = rng.choice([1, 10, 100], size=nq).astype(np.uint32)
+        L = np.maximum(K, rng.integers(lo, hi, size=nq)).astype(np.uint32)
+        print(f"longest-first, L in {lo}..{hi - 1}, {nq} queries", flush=True)
+        a = big["ix"].search(q, K, L, k_stride=100, pool_classes=SS_CLASSES_OFF)
+        b = big["ix"].search(q, K, L, k_stride=100, pool_classes=SS_CLASSES_OFF, query_order=SS_ORDER_INDEX)
+        _same(a, b, f"one kernel, L {lo}..{hi - 1}")
+        sel = rng.choice(nq, 120, replace=False)
+        o_ids, o_d, o_ndc, o_hops = oracle.search_batch(big["base"], big["adj"], big["ep"], q[sel], K[sel], L[sel],
+                                                        k_stride=100)
+        for j, i in enumerate(sel):
+            k = int(K[i])
+            assert np.array_equal(a[0][i, :k], o_ids[j, :k]), (lo, i)
+            assert np.array_equal(a[1][i, :k].view(np.uint32), o_d[j, :k].view(np.uint32)), (lo, i)
+        assert np.array_equal(a[2]["hops"][sel], o_hops)
+
+
 def test_locality_order_graph_replays(big):
     """hipGraph replays of the ordered launch (keys, sort, search), with eager
     work in between, each equal to the eager search."""
