@@ -616,23 +616,24 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         log(f"profile: {args.steps} mixed launches and 5 copies of 512 MiB done")
         return
 
-    # one kernel for every pool size, the class kernels one after another,
-    # then side by side (the default, whose figures are reported); all three
-    # return the same results
-    # The concurrent classes also run with one visited table for every class
-    # (slots, slots per bucket) beside the per-class tuned ones: the classes
-    # were tuned one at a time, and side by side they share the CUs; the
-    # fastest is the reported launch.  Every variant returns the same results.
+    # The class kernels side by side (the library's default), one after
+    # another, and one kernel for every pool size (longest pool first), each
+    # with the per-class tuned visited tables and with one table for every
+    # class (slots, slots per bucket): the classes were tuned one at a time,
+    # and side by side they share the CUs.  Every variant returns the same
+    # results; the fastest of the side-by-side and one-kernel launches is
+    # the reported one.
     from scattersearch_amd._lib import SS_CLASSES_AUTO, SS_CLASSES_OFF, SS_CLASSES_SERIAL
     variants = {}
     ref_ids = None
     best = None
     tables = [(args.visited_slots, args.visited_bucket)]
     if not args.visited_slots:
-        tables += [(4096, 8), (4096, 16), (2048, 16)]
-    runs = [("one_kernel", SS_CLASSES_OFF, tables[0]), ("classes_serial", SS_CLASSES_SERIAL, tables[0])]
-    runs += [("classes_concurrent" + ("" if tb == tables[0] else f"_slots{tb[0]}_bucket{tb[1]}"), SS_CLASSES_AUTO, tb)
-             for tb in tables]
+        tables += [(8192, 16), (4096, 16)]
+    runs = [("classes_serial", SS_CLASSES_SERIAL, tables[0])]
+    for tb in tables:
+        sfx = "" if tb == tables[0] else f"_slots{tb[0]}_bucket{tb[1]}"
+        runs += [("classes_concurrent" + sfx, SS_CLASSES_AUTO, tb), ("one_kernel" + sfx, SS_CLASSES_OFF, tb)]
     for name, mode, (slots_v, bucket_v) in runs:
         wall_v, kern_v, ids, st = timed(mode, slots_v, bucket_v)
         variants[name] = round(kern_v, 4)
@@ -641,9 +642,9 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
             ref_ids, ref_hops = ids.copy(), st["hops"].copy()
         elif not np.array_equal(ids, ref_ids) or not np.array_equal(st["hops"], ref_hops):
             raise RuntimeError(f"the mixed launch ({name}) returned other results than one kernel")
-        if mode == SS_CLASSES_AUTO and (best is None or kern_v < best[2]):
-            best = (name, wall_v, kern_v, ids, st, slots_v, bucket_v)
-    chosen, wall, kern_ms, ids, st, slots_c, bucket_c = best
+        if mode != SS_CLASSES_SERIAL and (best is None or kern_v < best[2]):
+            best = (name, wall_v, kern_v, ids, st, slots_v, bucket_v, mode)
+    chosen, wall, kern_ms, ids, st, slots_c, bucket_c, mode_c = best
     rec = {}
     for k in (1, 10, 100):
         sel = Ks == k
@@ -672,7 +673,9 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         "config": {"workload": w.name, "num": ix.info()["num"], "dim": w.dim, "nq": nq, "R": w.R,
                    "metric": "L2" if w.metric == 0 else "inner_product",
                    "K_mix": {str(k): int((Ks == k).sum()) for k in (1, 10, 100)}, "L_mean": float(Ls.mean()),
-                   "launch": {"pool_classes": "concurrent", "variant": chosen,
+                   "launch": {"pool_classes": ("concurrent" if mode_c == SS_CLASSES_AUTO
+                                               else "off (one kernel, longest pool first)"),
+                              "variant": chosen,
                               "visited_slots": slots_c or "tuned per class",
                               "visited_bucket": bucket_c or "tuned per class",
                               "classes": {f"L<={b}": {k: t[k] for k in ("waves_per_block", "visited_slots",

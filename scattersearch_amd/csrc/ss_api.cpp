@@ -622,7 +622,8 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         auto wave_bytes = [&](uint32_t n, uint32_t *mode, uint32_t *rem, uint32_t *mask) {
             *rem = *mask = 0;
             *mode = ss::hash_params16(ix->num, n, lbucket, rem, mask) ? (lbucket == 16 ? 2u : 1u) : 0u;
-            return ss::large_lds_bytes(ix->dim_pad, l_max, *mode ? n / 2 : n);
+            const size_t b = ss::large_lds_bytes(ix->dim_pad, l_max, *mode ? n / 2 : n);
+            return (b + ss::kLdsGranule - 1) / ss::kLdsGranule * ss::kLdsGranule;  // one wavefront per block
         };
         uint32_t mode = 0, rem = 0, mask = 0;
         size_t per_wave = wave_bytes(slots, &mode, &rem, &mask);
@@ -682,22 +683,35 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_OK;
     }
     s->hash_rem_bits = s->hash_id_mask = 0;
-    s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu,
-                               ss::search_reg_waves_per_cu(ix->metric, ix->dim_pad, ew, l_max, team, s->pool_lds),
-                               bucket, &s->hash_rem_bits, &s->hash_id_mask);
+    const uint32_t reg_cu = ss::search_reg_waves_per_cu(ix->metric, ix->dim_pad, ew, l_max, team, s->pool_lds);
+    s->hash16 = ss::use_hash16(ix->num, ix->dim_pad, l_max, slots, ew, ix->lds_per_cu, reg_cu, bucket,
+                               &s->hash_rem_bits, &s->hash_id_mask);
     const size_t per_wave = ss::search_lds_bytes_per_wave(ix->dim_pad, l_max, s->hash16 ? slots / 2 : slots, ew);
-    const uint32_t waves_cu = (uint32_t)std::min<size_t>(lds_cu / per_wave, 32);
-    if (waves_cu == 0) {
+    // Wavefronts per CU with blocks of w wavefronts: a block's LDS is
+    // allocated in granules, and the kernel's registers bound the
+    // wavefronts a SIMD holds.
+    auto resident = [&](uint32_t w) -> uint32_t {
+        const size_t blk = (w * per_wave + ss::kLdsGranule - 1) / ss::kLdsGranule * ss::kLdsGranule;
+        const uint32_t b = (uint32_t)std::min<size_t>(lds_cu / blk, std::max<uint32_t>(reg_cu, w) / w);
+        return std::min<uint32_t>(b * w, 32);
+    };
+    if (resident(1) == 0) {
         set_error("search state does not fit in LDS: lower L or visited_slots");
         return SS_EINVAL;
     }
-    // default: the largest of 4, 2, 1 wavefronts per block that divides the
-    // wavefronts a CU holds, so that whole blocks use all of them
-    uint32_t wpb = p && p->waves_per_block ? p->waves_per_block : (waves_cu % 4 == 0 ? 4u : (waves_cu % 2 == 0 ? 2u : 1u));
-    if (wpb == 0 || wpb > 8 || wpb > waves_cu) {
+    // default: the largest of 4, 2, 1 wavefronts per block that keeps the
+    // most wavefronts resident
+    uint32_t wpb = p && p->waves_per_block ? p->waves_per_block : 0;
+    if (!wpb) {
+        for (uint32_t w : {4u, 2u, 1u})
+            if (!wpb || resident(w) > resident(wpb))
+                wpb = w;
+    }
+    if (wpb > 8 || resident(wpb) == 0) {
         set_error("waves_per_block must be 1..8 and fit in LDS");
         return SS_EINVAL;
     }
+    const uint32_t waves_cu = resident(wpb);
     const uint32_t blocks_cu = waves_cu / wpb;
     uint32_t blocks = p && p->blocks ? p->blocks : (uint32_t)ix->num_cus * blocks_cu;
     const uint32_t need = (nq + wpb - 1) / wpb;

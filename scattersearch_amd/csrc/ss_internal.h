@@ -12,6 +12,8 @@ namespace ss {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kExplored = 0x80000000u;
+// LDS is allocated to a workgroup in granules of this many bytes.
+constexpr size_t kLdsGranule = 512;
 constexpr uint32_t kMaxProbe = 16;
 
 struct SearchArgs {
