@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--visited-bucket", type=int, default=0, help="16-bit visited-table bucket (8 or 16); 0 = tuned")
     ap.add_argument("--team-size", type=int, default=0, help="lanes per distance (4 or 8); 0 = tuned")
     ap.add_argument("--pool-layout", type=int, default=0, help="SS_POOL_* (1 registers, 2 LDS); 0 = tuned")
+    ap.add_argument("--pool-classes", choices=["auto", "off", "serial"], default="auto",
+                    help="--profile of the mixed launch: the class kernels side by side (auto), one kernel "
+                         "longest pool first (off), or the class kernels one after another (serial)")
     ap.add_argument("--query-order", choices=["auto", "index", "locality"], default="auto",
                     help="order in which the batched search takes the queries (results do not depend on it)")
     ap.add_argument("--graph-cache", default="",
@@ -461,11 +464,12 @@ def exact_ndc(ids, st, launch_with_slots, d_ids, stats_host):
     return best
 
 
-def measured_traffic(workload, nq, K, L, order):
+def measured_traffic(workload, nq, K, L, order, match=None):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
     calibrated against a plain copy) for this workload and launch, from the
     record kept in profiles/search_kernel_<workload>_counters.md; None when
-    there is no record for exactly this launch."""
+    there is no record for exactly this launch (workload, batch, K, L,
+    query order and the fields of `match`)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"search_kernel_{workload}_counters.md")
     try:
@@ -482,11 +486,13 @@ def measured_traffic(workload, nq, K, L, order):
     if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L"), rec.get("query_order", "index")) != \
             (workload, nq, K, L, order):
         return None
+    if any(rec.get(k) != v for k, v in (match or {}).items()):
+        return None
     return rec
 
 
-def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_ms):
-    rec = measured_traffic(workload, nq, K, L, order)
+def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_ms, match=None):
+    rec = measured_traffic(workload, nq, K, L, order, match)
     out = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
            "traffic": rec["traffic_bytes"] if rec else None,
@@ -604,10 +610,11 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         return wall_, float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs])), ids_, st_
 
     if args.profile:
-        from scattersearch_amd._lib import SS_CLASSES_AUTO
+        from scattersearch_amd._lib import SS_CLASSES_AUTO, SS_CLASSES_OFF, SS_CLASSES_SERIAL
+        mode = {"auto": SS_CLASSES_AUTO, "off": SS_CLASSES_OFF, "serial": SS_CLASSES_SERIAL}[args.pool_classes]
         with torch.cuda.stream(stream):
             for _ in range(args.steps):
-                launch(SS_CLASSES_AUTO, args.visited_slots, args.visited_bucket)
+                launch(mode, args.visited_slots, args.visited_bucket)
             src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
             dst = torch.empty_like(src)
             for _ in range(5):
@@ -662,6 +669,8 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
     achieved = launch_bytes / (kern_ms * 1e-3) / 1e9
     khz = ix.wall_clock_khz()
     service_ms = (st["t_end"].astype(np.float64) - st["t_start"]) / khz
+    start_ms = (st["t_start"].astype(np.float64) - float(st["t_start"].min())) / khz
+    slow = int(np.argmax(service_ms))
     qps = nq * args.steps / wall
     log(f"mixed launch: {qps:.0f} QPS, {kern_ms:.3f} ms per launch, {achieved:.0f} GB/s algorithmic, recall {rec}")
     out = {
@@ -688,9 +697,16 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
         "recomputed_ndc": round(float((st["ndc"] - ndc_exact).mean()), 1),
         "latency_ms": {"service_p50": round(metrics.percentile(service_ms, 0.5), 4),
                        "service_p99": round(metrics.percentile(service_ms, 0.99), 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": round(launch_bytes), "kernel_ms": round(kern_ms, 4)},
+        # the launch cannot end before its slowest search: a chain of hops
+        # one after another
+        "slowest_search": {"L": int(Ls[slow]), "K": int(Ks[slow]), "hops": int(st["hops"][slow]),
+                           "ndc": int(ndc_exact[slow]), "service_ms": round(float(service_ms[slow]), 4),
+                           "started_ms": round(float(start_ms[slow]), 4)},
+        # the counters were read on one kernel longest pool first; the class
+        # kernels side by side run the same searches (same distance counts
+        # with the same visited table), so the record serves both
+        "roofline": roofline_record(w.name, nq, "mixed", "mixed", "longest_first", achieved, launch_bytes, kern_ms,
+                                    match={"visited_slots": slots_c, "visited_bucket": bucket_c}),
         "graph": {"mean_degree": ginfo["passes"][-1]["mean_degree"], "knn_s": round(ginfo["knn_s"], 1),
                   "passes": len(ginfo["passes"])},
     }
