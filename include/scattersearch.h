@@ -78,12 +78,15 @@ typedef struct ss_search_params {
                                  exact parity; expand_width must be 1 */
     uint32_t query_order;     /* order in which the batched search takes the
                                  queries (results do not depend on it):
-                                 SS_ORDER_AUTO (0): SS_ORDER_LOCALITY for
-                                 batches of at least SS_ORDER_MIN_BATCH queries
-                                 with one wavefront per query and one pool
-                                 class (see pool_classes), longest pool first
-                                 within each class for a launch split into
-                                 several, else index order; SS_ORDER_INDEX
+                                 SS_ORDER_AUTO (0): for batches of at least
+                                 SS_ORDER_MIN_BATCH queries with one
+                                 wavefront per query, longest pool first when
+                                 per-query L goes above 64 (within each class
+                                 for a launch split into several, over the
+                                 whole batch for one kernel), else
+                                 SS_ORDER_LOCALITY; longest pool first within
+                                 each class for a smaller split launch; else
+                                 index order; SS_ORDER_INDEX
                                  (1): no reordering beyond the grouping by
                                  class; SS_ORDER_LOCALITY
                                  (2): queries near one another in space are
