@@ -256,21 +256,24 @@ def main():
         chosen = L_list[-1]
         log(f"no L reached recall {args.recall_target}; reporting the largest L")
     elif not args.L and args.workload == "c1":
-        # refine between the last L below the target and the first above it
+        # the smallest L that reaches the target: refine between the last L
+        # below it and the first above it in steps of 8, 2, then 1
         lo = max([r["L"] for r in sweep if r["L"] < chosen] or [K])
-        for Lr in range(((lo + 8) // 8) * 8, chosen, 8):
-            wall, kern_ms = timed(Lr, 3, 1)
-            ids = d_ids.cpu().numpy().view(np.uint32)
-            rec = recall_of(ids)
-            st = stats_host()
-            row = dict(L=Lr, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
-                       mean_ndc=round(float(st["ndc"].mean()), 1), mean_hops=round(float(st["hops"].mean()), 1),
-                       lossy=int(st["lossy"].sum()))
-            sweep.append(row)
-            log(f"refine {row}")
-            if rec >= args.recall_target:
-                chosen = Lr
-                break
+        for step in (8, 2, 1):
+            for Lr in range(lo + step, chosen, step):
+                wall, kern_ms = timed(Lr, 3, 1)
+                ids = d_ids.cpu().numpy().view(np.uint32)
+                rec = recall_of(ids)
+                st = stats_host()
+                row = dict(L=Lr, recall=round(rec, 4), qps=round(nq * 3 / wall, 1), kernel_ms=round(kern_ms, 3),
+                           mean_ndc=round(float(st["ndc"].mean()), 1),
+                           mean_hops=round(float(st["hops"].mean()), 1), lossy=int(st["lossy"].sum()))
+                sweep.append(row)
+                log(f"refine {row}")
+                if rec >= args.recall_target:
+                    chosen = Lr
+                    break
+                lo = Lr
     L = chosen
     sweep.sort(key=lambda r: r["L"])
 
