@@ -84,6 +84,47 @@ def test_longest_first_order_of_one_kernel(big):
         assert np.array_equal(a[2]["hops"][sel], o_hops)
 
 
+def test_longest_first_order_without_allocation(big):
+    """The longest-first order of one kernel also runs on the no-allocation
+    entry point (keys and order in its scratch array): same results as
+    ss_search_device, for both pool kernels."""
+    import ctypes
+    import torch
+    from scattersearch_amd import dim_padded
+    from scattersearch_amd._lib import SS_CLASSES_OFF, SS_ORDER_MIN_BATCH, SearchParams, lib
+    ix, q = big["ix"], big["queries"]
+    nq = SS_ORDER_MIN_BATCH + 500
+    rng = np.random.default_rng(17)
+    dp = dim_padded(q.shape[1])
+    dq = torch.zeros((nq, dp), dtype=torch.float32, device="cuda")
+    dq[:, :q.shape[1]] = torch.from_numpy(q[:nq]).cuda()
+    vp = ctypes.c_void_p
+    for lo, hi in ((65, 300), (513, 700)):
+        K = rng.choice([1, 10, 100], size=nq).astype(np.int32)
+        L = np.maximum(K, rng.integers(lo, hi + 1, size=nq)).astype(np.int32)
+        dK = torch.from_numpy(K).cuda()
+        dL = torch.from_numpy(L).cuda()
+        out = []
+        for fn in (lib.ss_search_device, lib.ss_search_device_noalloc):
+            print(f"longest-first without allocation: L {lo}..{hi}, {fn.__name__}", flush=True)
+            p = SearchParams()
+            p.pool_classes = SS_CLASSES_OFF
+            ids = torch.full((nq, 100), -1, dtype=torch.int32, device="cuda")
+            ds = torch.zeros((nq, 100), dtype=torch.float32, device="cuda")
+            st = torch.zeros((nq, 8), dtype=torch.int32, device="cuda")
+            rc = fn(ix.handle, vp(dq.data_ptr()), nq, vp(dK.data_ptr()), vp(dL.data_ptr()), 100, hi, 100,
+                    ctypes.byref(p), vp(ids.data_ptr()), vp(ds.data_ptr()), vp(st.data_ptr()), None)
+            torch.cuda.synchronize()
+            assert rc == 0, lib.ss_last_error()
+            out.append((ids.cpu().numpy(), ds.cpu().numpy().view(np.uint32), st.cpu().numpy()))
+        (ia, da, sa), (ib, db, sb) = out
+        for i in range(nq):
+            k = int(K[i])
+            assert np.array_equal(ia[i, :k], ib[i, :k]), (lo, i)
+            assert np.array_equal(da[i, :k], db[i, :k]), (lo, i)
+        assert np.array_equal(sa[:, :4], sb[:, :4]), lo  # ndc, hops, lossy, status
+
+
 def test_locality_order_graph_replays(big):
     """hipGraph replays of the ordered launch (keys, sort, search), with eager
     work in between, each equal to the eager search."""
