@@ -292,9 +292,11 @@ int ss_nn_descent(ss_index *index, uint32_t k, uint32_t iters, uint32_t seed, ui
 /* Autotuner: times launch configurations for pool size L on a sample of
  * queries and keeps the fastest whose recall@K against exact brute force on
  * the sample is within max_recall_loss of the exact search: first waves
- * per block and visited-hash size (and, with a recall allowance, expansion
- * width and wavefronts per query), then distance team size and pool layout
- * at the best of those (these two never change results).  max_recall_loss
+ * per block and visited-hash size for both distance team sizes (and, with
+ * a recall allowance, expansion width and wavefronts per query), then the
+ * pool layouts at the best of those (neither team size nor layout changes
+ * results), then waves per block, visited-hash size and slots per bucket
+ * again with the team size and layout chosen.  max_recall_loss
  * = 0 keeps exact results.  The choice is stored in the index for L's
  * power-of-two bucket (dimension and degree are the index's) and used by
  * later searches that leave those fields 0. */

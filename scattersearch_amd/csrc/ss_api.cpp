@@ -1658,8 +1658,10 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
     };
     std::vector<Cand> done;
     // Stage 1 candidates: exact one-wavefront search over waves per block and
-    // visited slots (four-lane teams, the default pool layout); with a recall
-    // allowance also wider expansion and several wavefronts per query.
+    // visited slots, with distance teams of 4 and of 8 lanes (the default
+    // pool layout; the teams' registers differ, so do their best block
+    // shapes); with a recall allowance also wider expansion and several
+    // wavefronts per query.
     const uint32_t dflt_layout = L <= 64 ? SS_POOL_REGISTERS : SS_POOL_LDS;
     std::vector<ss_search_params> cands;
     const uint32_t wpbs[4] = {1, 2, 4, 8};
@@ -1667,18 +1669,22 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
     for (uint32_t ew : {1u, 2u, 4u}) {
         if (ew > 1 && !(max_recall_loss > 0.0f))
             continue;
-        for (uint32_t slots : slotss)
-            for (uint32_t wpb : wpbs) {
-                ss_search_params p{};
-                p.expand_width = ew;
-                p.waves_per_block = wpb;
-                p.visited_slots = slots;
-                p.waves_per_query = 1;
-                p.team_size = 4;
-                p.pool_layout = dflt_layout;
-                p.visited_bucket = 8;
-                cands.push_back(p);
-            }
+        for (uint32_t team : {4u, 8u}) {
+            if (team == 8 && ew != 1)  // teams of 8 run the one-node expansion only
+                continue;
+            for (uint32_t slots : slotss)
+                for (uint32_t wpb : wpbs) {
+                    ss_search_params p{};
+                    p.expand_width = ew;
+                    p.waves_per_block = wpb;
+                    p.visited_slots = slots;
+                    p.waves_per_query = 1;
+                    p.team_size = team;
+                    p.pool_layout = dflt_layout;
+                    p.visited_bucket = 8;
+                    cands.push_back(p);
+                }
+        }
     }
     if (max_recall_loss > 0.0f)
         for (uint32_t wq : {2u, 4u})
@@ -1744,7 +1750,7 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
         const ss_search_params base_p = b1->p;
         for (uint32_t team : {4u, 8u})
             for (uint32_t layout : {SS_POOL_REGISTERS, SS_POOL_LDS, SS_POOL_SHIFT}) {
-                if ((layout == SS_POOL_REGISTERS && L > 64) || (team == 4 && layout == dflt_layout) ||
+                if ((layout == SS_POOL_REGISTERS && L > 64) || (team == base_p.team_size && layout == dflt_layout) ||
                     (team == 8 && layout == SS_POOL_SHIFT))  // the shift-merge kernel has teams of 4
                     continue;
                 ss_search_params p = base_p;
@@ -1754,6 +1760,32 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
                 if (rc)
                     break;
             }
+    }
+    // Stage 3: waves per block, visited slots and slots per bucket again
+    // with the team size and pool layout stage 2 chose (the best block shape
+    // depends on the registers and LDS of the kernel); shapes already timed
+    // are not timed again.
+    const Cand *b2 = rc ? nullptr : pick();
+    if (b2 && b2->p.expand_width == 1 && b2->p.waves_per_query == 1) {
+        const ss_search_params base_p = b2->p;
+        for (uint32_t slots : slotss)
+            for (uint32_t bucket : {8u, 16u})
+                for (uint32_t wpb : wpbs) {
+                    if ((bucket == 16 && slots < 2048) || (base_p.pool_layout == SS_POOL_SHIFT && wpb != 1))
+                        continue;
+                    ss_search_params p = base_p;
+                    p.visited_slots = slots;
+                    p.visited_bucket = bucket;
+                    p.waves_per_block = wpb;
+                    bool seen = false;
+                    for (const Cand &c : done)
+                        seen = seen || std::memcmp(&c.p, &p, sizeof(p)) == 0;
+                    if (seen)
+                        continue;
+                    measure(p);
+                    if (rc)
+                        break;
+                }
     }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
