@@ -639,10 +639,11 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
     best = None
     tables = [(args.visited_slots, args.visited_bucket)]
     if not args.visited_slots:
-        tables += [(8192, 16), (4096, 16)]
+        tables += [(8192, 16), (8192, 16 | ss._lib.SS_VISITED_TWO_CHOICE), (4096, 16)]
     runs = [("classes_serial", SS_CLASSES_SERIAL, tables[0])]
     for tb in tables:
-        sfx = "" if tb == tables[0] else f"_slots{tb[0]}_bucket{tb[1]}"
+        two = tb[1] & ss._lib.SS_VISITED_TWO_CHOICE
+        sfx = "" if tb == tables[0] else f"_slots{tb[0]}_bucket{tb[1] & 0xFF}" + ("_two_choice" if two else "")
         runs += [("classes_concurrent" + sfx, SS_CLASSES_AUTO, tb), ("one_kernel" + sfx, SS_CLASSES_OFF, tb)]
     for name, mode, (slots_v, bucket_v) in runs:
         wall_v, kern_v, ids, st = timed(mode, slots_v, bucket_v)
@@ -689,7 +690,8 @@ def mixed_run(args, w, ix, queries, ginfo, r_pad):
                                                else "off (one kernel, longest pool first)"),
                               "variant": chosen,
                               "visited_slots": slots_c or "tuned per class",
-                              "visited_bucket": bucket_c or "tuned per class",
+                              "visited_bucket": ((f"{bucket_c & 0xFF}" + (" two-choice" if bucket_c & 0x100 else ""))
+                                                 if bucket_c else "tuned per class"),
                               "classes": {f"L<={b}": {k: t[k] for k in ("waves_per_block", "visited_slots",
                                                                           "visited_bucket", "team_size",
                                                                           "pool_layout")}

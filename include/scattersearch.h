@@ -128,9 +128,16 @@ typedef struct ss_search_params {
                                  LDS read per lookup) or 16 (two reads, far
                                  fewer full buckets, so the same ids fit a
                                  smaller table and more wavefronts fit a CU);
+                                 either | SS_VISITED_TWO_CHOICE: an id may
+                                 also go to a second bucket (twice the reads
+                                 per lookup; a table about 3/4 full almost
+                                 never forgets an id), where the ids leave a
+                                 bit of the slot free, else one bucket;
                                  0 = 8.  Results do not depend on it (a full
                                  bucket only recomputes a distance) */
 } ss_search_params;
+
+#define SS_VISITED_TWO_CHOICE 0x100u
 
 #define SS_CLASSES_AUTO 0u
 #define SS_CLASSES_OFF 1u
@@ -295,8 +302,9 @@ int ss_nn_descent(ss_index *index, uint32_t k, uint32_t iters, uint32_t seed, ui
  * per block and visited-hash size for both distance team sizes (and, with
  * a recall allowance, expansion width and wavefronts per query), then the
  * pool layouts at the best of those (neither team size nor layout changes
- * results), then waves per block, visited-hash size and slots per bucket
- * again with the team size and layout chosen.  max_recall_loss
+ * results), then waves per block, visited-hash size and buckets (8 or 16
+ * slots, one or two per id) again with the team size and layout chosen.
+ * max_recall_loss
  * = 0 keeps exact results.  The choice is stored in the index for L's
  * power-of-two bucket (dimension and degree are the index's) and used by
  * later searches that leave those fields 0. */

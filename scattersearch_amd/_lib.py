@@ -45,6 +45,8 @@ SS_CLASSES_AUTO = 0
 SS_CLASSES_OFF = 1
 SS_CLASSES_SERIAL = 2
 
+SS_VISITED_TWO_CHOICE = 0x100
+
 
 class SearchParams(ctypes.Structure):
     _fields_ = [

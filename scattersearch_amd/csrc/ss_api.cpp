@@ -519,7 +519,8 @@ int ss_load_graph(ss_index *ix, const char *path)
 struct Shape {
     uint32_t wpb, slots, blocks;
     size_t lds_wave;
-    uint32_t hash16;  // 16-bit visited slots (one wavefront per query): 1 buckets of 8, 2 of 16
+    uint32_t hash16;  // 16-bit visited slots (one wavefront per query): 1 buckets of 8, 2 of 16, +4 two
+                      // buckets per id
     uint32_t hash_rem_bits, hash_id_mask;
     uint32_t team, pool_lds;
 };
@@ -590,8 +591,8 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         return SS_EINVAL;
     }
     const uint32_t bucket = p && p->visited_bucket ? p->visited_bucket : 8;
-    if (bucket != 8 && bucket != 16) {
-        set_error("visited_bucket must be 8 or 16");
+    if ((bucket & ~SS_VISITED_TWO_CHOICE) != 8 && (bucket & ~SS_VISITED_TWO_CHOICE) != 16) {
+        set_error("visited_bucket must be 8 or 16, optionally with SS_VISITED_TWO_CHOICE");
         return SS_EINVAL;
     }
     const uint32_t layout = p ? p->pool_layout : SS_POOL_AUTO;
@@ -621,7 +622,7 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
         const uint32_t lbucket = (p && p->visited_bucket) ? bucket : (l_max > SS_MAX_L ? 16u : bucket);
         auto wave_bytes = [&](uint32_t n, uint32_t *mode, uint32_t *rem, uint32_t *mask) {
             *rem = *mask = 0;
-            *mode = ss::hash_params16(ix->num, n, lbucket, rem, mask) ? (lbucket == 16 ? 2u : 1u) : 0u;
+            *mode = ss::hash16_mode(ix->num, n, lbucket, rem, mask);
             const size_t b = ss::large_lds_bytes(ix->dim_pad, l_max, *mode ? n / 2 : n);
             return (b + ss::kLdsGranule - 1) / ss::kLdsGranule * ss::kLdsGranule;  // one wavefront per block
         };
@@ -726,13 +727,13 @@ static int choose_shape(const ss_index *ix, uint32_t l_max, uint32_t nq, const s
 }  // extern "C"
 
 namespace ss {
-bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask)
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask, bool two)
 {
     uint32_t sb = 0;  // log2(slots)
     while ((1u << sb) < slots)
         sb++;
     const uint32_t lb = bucket == 16 ? 4u : 3u;  // log2(slots per bucket)
-    if (sb < lb)
+    if (sb < lb || (two && sb == lb))  // two choices need two buckets
         return false;
     const uint32_t bb = sb - lb;  // log2(buckets)
     uint32_t n = 0;  // ceil(log2(num))
@@ -740,11 +741,20 @@ bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_
         n++;
     if (n < bb)
         n = bb;
-    if (n > 31 || n - bb > 15)
+    if (n > 31 || n - bb > (two ? 14u : 15u))  // two choices keep a flag in bit 14
         return false;
     *rem_bits = n - bb;
     *id_mask = (1u << n) - 1u;
     return true;
+}
+
+uint32_t hash16_mode(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask)
+{
+    const uint32_t size = bucket & ~SS_VISITED_TWO_CHOICE;
+    const uint32_t base = size == 16 ? 2u : 1u;
+    if ((bucket & SS_VISITED_TWO_CHOICE) && hash_params16(num, slots, size, rem_bits, id_mask, true))
+        return base | 4u;
+    return hash_params16(num, slots, size, rem_bits, id_mask, false) ? base : 0u;
 }
 
 uint32_t use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slots, uint32_t expand_width,
@@ -752,15 +762,15 @@ uint32_t use_hash16(uint32_t num, uint32_t dim_pad, uint32_t l_max, uint32_t slo
 {
     if (expand_width != 1)  // the wider-expansion kernels keep the 32-bit table
         return 0;
-    if (bucket == 16)  // asked for by name: the 16-bit table wherever the ids fit it
-        return hash_params16(num, slots, 16, rem_bits, id_mask) ? 2u : 0u;
+    if (bucket != 8)  // asked for by name: the 16-bit table wherever the ids fit it
+        return hash16_mode(num, slots, bucket, rem_bits, id_mask);
     const size_t lds_cu = lds_per_cu ? lds_per_cu : 65536;
     const size_t w32 = std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots, 1), reg_waves_cu);
     const size_t w16 =
         std::min<size_t>(lds_cu / search_lds_bytes_per_wave(dim_pad, l_max, slots / 2, 1), reg_waves_cu);
     if (w16 <= w32)
         return 0;
-    return hash_params16(num, slots, 8, rem_bits, id_mask) ? 1u : 0u;
+    return hash_params16(num, slots, 8, rem_bits, id_mask, false) ? 1u : 0u;
 }
 
 int index_view(const ss_index *ix, IndexView *v)
@@ -1769,9 +1779,9 @@ int ss_autotune(ss_index *ix, const float *queries, uint32_t nq, uint32_t K, uin
     if (b2 && b2->p.expand_width == 1 && b2->p.waves_per_query == 1) {
         const ss_search_params base_p = b2->p;
         for (uint32_t slots : slotss)
-            for (uint32_t bucket : {8u, 16u})
+            for (uint32_t bucket : {8u, 16u, 8u | SS_VISITED_TWO_CHOICE, 16u | SS_VISITED_TWO_CHOICE})
                 for (uint32_t wpb : wpbs) {
-                    if ((bucket == 16 && slots < 2048) || (base_p.pool_layout == SS_POOL_SHIFT && wpb != 1))
+                    if ((bucket != 8 && slots < 2048) || (base_p.pool_layout == SS_POOL_SHIFT && wpb != 1))
                         continue;
                     ss_search_params p = base_p;
                     p.visited_slots = slots;

@@ -31,9 +31,11 @@ struct SearchArgs {
     uint32_t l_max;       // sizes the LDS pool (<= SS_MAX_L)
     uint32_t hash_shift;  // 32 - log2(hash slots)
     uint32_t hash_slots;
-    uint32_t hash16;        // 16-bit visited slots (hash_params16), one wavefront per query only:
-                            // 1 in buckets of 8 slots, 2 in buckets of 16; 0: 32-bit slots
-    uint32_t hash_rem_bits; // 16-bit slots: bits of the hashed id kept in a slot (<= 15)
+    uint32_t hash16;        // 16-bit visited slots (hash16_mode), one wavefront per query only:
+                            // 1 in buckets of 8 slots, 2 in buckets of 16, + 4 with two
+                            // buckets per id; 0: 32-bit slots
+    uint32_t hash_rem_bits; // 16-bit slots: bits of the hashed id kept in a slot (<= 15; <= 14
+                            // with two buckets per id)
     uint32_t hash_id_mask;  // 16-bit slots: ids are hashed modulo this + 1 (a power of two)
     uint32_t k_stride;
     uint32_t q_first;     // first query index this launch handles
@@ -296,7 +298,13 @@ int index_view(const ss_index *ix, IndexView *v);
 // (id * 0x9E3779B1) mod 2^n is a bijection whose top b bits pick the bucket and
 // whose low n - b bits are stored, which needs n - b <= 15 (bit 15 marks an
 // empty slot).
-bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask);
+bool hash_params16(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask,
+                   bool two);
+// Mode of SearchArgs.hash16 for a table of `slots` 16-bit slots and a
+// visited_bucket value (8 or 16, optionally | SS_VISITED_TWO_CHOICE): 1
+// (buckets of 8) or 2 (of 16), plus 4 for two choices when the ids leave
+// bit 14 free (else one choice); 0 when the ids do not fit 16-bit slots.
+uint32_t hash16_mode(uint32_t num, uint32_t slots, uint32_t bucket, uint32_t *rem_bits, uint32_t *id_mask);
 // Which visited table a one-wavefront-per-query launch (one node per step)
 // uses: 2 (16-bit slots in buckets of 16) when bucket = 16 asks for it and the
 // ids fit (hash_params16); with bucket = 8, 1 (16-bit slots in buckets of 8)

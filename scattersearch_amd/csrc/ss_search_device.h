@@ -490,6 +490,57 @@ __device__ __forceinline__ bool visit16_begin(uint32_t *tab, uint32_t rem_bits, 
     return true;
 }
 
+// Two-choice form (an id may also live in a second bucket, so that a table
+// loaded up to about 3/4 almost never finds both full): the second bucket is
+// b ^ g(rem), with g(rem) odd and below the bucket count, and a slot there
+// keeps rem with bit 14 set (rem_bits <= 14), so a slot value and its bucket
+// still give the id back: bit 14 says which bucket is the id's first one.
+// A lookup reads both buckets; an insert goes to the one with more free
+// slots (the first on a tie).  visit16_settle completes its claim unchanged.
+__device__ __forceinline__ bool visit16_begin2(uint32_t *tab, uint32_t rem_bits, uint32_t id_mask, bool wide,
+                                               uint32_t id, VisitClaim &c, uint32_t &lossy)
+{
+    c.slot = kNone;
+    c.old = kNone;
+    const uint32_t h = (id * 0x9E3779B1u) & id_mask;
+    const uint32_t b = h >> rem_bits;
+    const uint32_t rem = h & ((1u << rem_bits) - 1u);
+    const uint32_t g = (((rem * 0x85EBCA6Bu) >> 16) & (id_mask >> rem_bits)) | 1u;
+    const uint32_t b2 = b ^ g;
+    const uint32_t alt = rem | 0x4000u;
+    const uint32_t rr = rem | (rem << 16), ra = alt | (alt << 16);
+    const uint4 *t4 = reinterpret_cast<const uint4 *>(tab);
+    uint32_t fm, fm2, nslot;
+    if (wide) {
+        const uint4 x = t4[2u * b], y = t4[2u * b + 1u], x2 = t4[2u * b2], y2 = t4[2u * b2 + 1u];
+        if (zero_half16(x, rr) | zero_half16(y, rr) | zero_half16(x2, ra) | zero_half16(y2, ra))
+            return false;
+        fm = free_mask16(x) | (free_mask16(y) << 8);
+        fm2 = free_mask16(x2) | (free_mask16(y2) << 8);
+        nslot = 2u * kBucket;
+    } else {
+        const uint4 x = t4[b], x2 = t4[b2];
+        if (zero_half16(x, rr) | zero_half16(x2, ra))
+            return false;
+        fm = free_mask16(x);
+        fm2 = free_mask16(x2);
+        nslot = kBucket;
+    }
+    const bool second = __builtin_popcount(fm2) > __builtin_popcount(fm);
+    const uint32_t f = second ? fm2 : fm, bk = second ? b2 : b, v = second ? alt : rem;
+    if (f == 0) {
+        lossy++;
+        return true;
+    }
+    // first free slot at or after slot (id mod bucket size), cyclically
+    const uint32_t s0 = id & (nslot - 1u);
+    const uint32_t f2 = f | (f << nslot);
+    c.slot = bk * nslot + ((s0 + (uint32_t)__builtin_ctz(f2 >> s0)) & (nslot - 1u));
+    c.old = v;
+    reinterpret_cast<uint16_t *>(tab)[c.slot] = (uint16_t)v;
+    return true;
+}
+
 __device__ __forceinline__ void visit16_settle(uint32_t *tab, bool wide, const VisitClaim &c, uint32_t &lossy)
 {
     if (c.slot == kNone)
@@ -515,8 +566,9 @@ __device__ __forceinline__ bool visit_any(const SearchArgs &a, uint32_t *tab, ui
 {
     if (a.hash16) {
         VisitClaim c;
-        const bool wide = a.hash16 == 2u;
-        const bool fresh = visit16_begin(tab, a.hash_rem_bits, a.hash_id_mask, wide, id, c, lossy);
+        const bool wide = (a.hash16 & 3u) == 2u;
+        const bool fresh = (a.hash16 & 4u) ? visit16_begin2(tab, a.hash_rem_bits, a.hash_id_mask, wide, id, c, lossy)
+                                           : visit16_begin(tab, a.hash_rem_bits, a.hash_id_mask, wide, id, c, lossy);
         visit16_settle(tab, wide, c, lossy);
         return fresh;
     }
@@ -802,7 +854,8 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
     const int tail_rt = (int)((a.dim_pad % 16) != 0);
     const uint32_t bshift = a.hash_shift + 3u;  // buckets of eight slots
     const bool h16 = a.hash16 != 0;
-    const bool hwide = a.hash16 == 2u;
+    const bool hwide = (a.hash16 & 3u) == 2u;
+    const bool htwo = (a.hash16 & 4u) != 0;
     const uint32_t L = st.L;
 
     // k is the pool position of the node to expand and nb already holds its
@@ -922,8 +975,10 @@ __device__ __forceinline__ void expand_loop(const SearchArgs &a, const WaveLds &
         // Visited test of the whole row, compaction of the fresh ids.
         VisitClaim claim{kNone, kNone};
         const bool fresh =
-            nb != kNone && (h16 ? visit16_begin(w.hash, a.hash_rem_bits, a.hash_id_mask, hwide, nb, claim, st.lossy)
-                                : visit_begin(w.hash, bshift, nb, claim, st.lossy));
+            nb != kNone &&
+            (h16 ? (htwo ? visit16_begin2(w.hash, a.hash_rem_bits, a.hash_id_mask, hwide, nb, claim, st.lossy)
+                         : visit16_begin(w.hash, a.hash_rem_bits, a.hash_id_mask, hwide, nb, claim, st.lossy))
+                 : visit_begin(w.hash, bshift, nb, claim, st.lossy));
         const uint64_t fm = ballot(fresh);
         const uint32_t nc = (uint32_t)__popcll(fm);
         if (fresh)

@@ -96,7 +96,7 @@ def test_status_codes_match_the_header():
         assert f in hdr, f
     for name in ("SS_POOL_AUTO", "SS_POOL_REGISTERS", "SS_POOL_LDS", "SS_POOL_SHIFT", "SS_NOALLOC_MAX_QUERIES",
                  "SS_CLASSES_AUTO",
-                 "SS_CLASSES_OFF", "SS_CLASSES_SERIAL"):
+                 "SS_CLASSES_OFF", "SS_CLASSES_SERIAL", "SS_VISITED_TWO_CHOICE"):
         assert getattr(_lib, name) == defs[name], name
 
 

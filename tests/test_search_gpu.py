@@ -366,6 +366,63 @@ def test_wide_buckets_forget_less_than_narrow(c0):
     assert wide < narrow, (wide, narrow)
 
 
+@pytest.mark.parametrize("bucket", [8, 16])
+@pytest.mark.parametrize("slots", [256, 1024, 4096])
+@pytest.mark.parametrize("L", [48, 128, 512, 900])
+def test_two_choice_buckets_equal_oracle(c0, bucket, slots, L):
+    """The 16-bit table with a second bucket per id (SS_VISITED_TWO_CHOICE):
+    ids, distance bits and hops equal the oracle for both bucket sizes, at
+    every pool layout and in the large-pool kernel (L = 900); distance
+    counts equal it wherever nothing was forgotten and never fall below."""
+    import oracle
+    from scattersearch_amd._lib import SS_VISITED_TWO_CHOICE
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"]
+    gpu = ix.search(q, 10, L, visited_slots=slots, visited_bucket=bucket | SS_VISITED_TWO_CHOICE)
+    ora = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], q, 10, L)
+    _assert_same(gpu, ora, check_ndc=False, what=f"two-choice buckets of {bucket}, {slots} slots, L={L}")
+    st = gpu[2]
+    exact = st["lossy"] == 0
+    assert np.array_equal(st["ndc"][exact], ora[2][exact])
+    assert (st["ndc"] >= ora[2]).all()
+    if slots == 256 and L >= 128:
+        assert st["lossy"].sum() > 0
+
+
+def test_two_choice_buckets_forget_less_than_one(c0):
+    """At the same table size and load, a second bucket per id forgets far
+    fewer ids than one bucket."""
+    from scattersearch_amd._lib import SS_VISITED_TWO_CHOICE
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"]
+    for bucket in (8, 16):
+        one = ix.search(q, 10, 256, visited_slots=1024, visited_bucket=bucket)[2]["lossy"].sum()
+        two = ix.search(q, 10, 256, visited_slots=1024, visited_bucket=bucket | SS_VISITED_TWO_CHOICE)[2]
+        assert two["lossy"].sum() < one, (bucket, two["lossy"].sum(), one)
+
+
+def test_two_choice_buckets_in_one_launch_mixing_pool_sizes(small):
+    """Per-query K and L through the pool-size classes and through one
+    kernel with two-choice buckets equal the oracle."""
+    import oracle
+    from scattersearch_amd._lib import SS_CLASSES_AUTO, SS_CLASSES_OFF, SS_VISITED_TWO_CHOICE
+    rng = np.random.default_rng(29)
+    q = small["queries"]
+    nq = q.shape[0]
+    K = rng.choice([1, 10, 32], size=nq).astype(np.uint32)
+    L = np.array([rng.integers(max(int(k), 16), 513) for k in K], dtype=np.uint32)
+    ix = _index(small["base"], small["adj"], small["ep"])
+    ora = oracle.search_batch(small["base"], small["adj"], small["ep"], q, K, L, k_stride=32)
+    for mode in (SS_CLASSES_AUTO, SS_CLASSES_OFF):
+        gpu = ix.search(q, K, L, k_stride=32, visited_bucket=16 | SS_VISITED_TWO_CHOICE, visited_slots=2048,
+                        pool_classes=mode)
+        for i in range(nq):
+            k = int(K[i])
+            assert np.array_equal(gpu[0][i, :k], ora[0][i, :k]), f"mode {mode} query {i} K={k} L={L[i]}"
+            assert np.array_equal(gpu[1][i, :k].view(np.uint32), ora[1][i, :k].view(np.uint32))
+            assert gpu[2]["hops"][i] == ora[3][i]
+
+
 def test_wide_buckets_in_one_launch_mixing_pool_sizes(small):
     """Per-query K and L through the pool-size classes with 16-slot buckets
     equal the oracle."""
@@ -388,7 +445,9 @@ def test_wide_buckets_in_one_launch_mixing_pool_sizes(small):
 def test_visited_bucket_rejects_other_sizes(small):
     from scattersearch_amd import SSError
     from scattersearch_amd._lib import SS_EINVAL
+    from scattersearch_amd._lib import SS_VISITED_TWO_CHOICE
     ix = _index(small["base"], small["adj"], small["ep"])
-    with pytest.raises(SSError) as e:
-        ix.search(small["queries"][:4], 10, 48, visited_bucket=4)
-    assert e.value.code == SS_EINVAL
+    for bad in (4, 32, SS_VISITED_TWO_CHOICE, 4 | SS_VISITED_TWO_CHOICE, 16 | 0x200):
+        with pytest.raises(SSError) as e:
+            ix.search(small["queries"][:4], 10, 48, visited_bucket=bad)
+        assert e.value.code == SS_EINVAL, bad
