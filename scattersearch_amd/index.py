@@ -124,8 +124,8 @@ class Index:
         """Search a batch of host queries.  K and L are ints or per-query
         arrays.  query_order is SS_ORDER_* (scheduling only; results do not
         depend on it); team_size (4, 8), pool_layout (SS_POOL_*),
-        pool_classes (SS_CLASSES_*) and visited_bucket (8, 16) do not change
-        results either.  Returns (ids [nq][k_stride], dists, stats)."""
+        pool_classes (SS_CLASSES_*) and visited_bucket (8 or 16, optionally
+        | SS_VISITED_TWO_CHOICE) do not change results either.  Returns (ids [nq][k_stride], dists, stats)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 1:
             q = q[None, :]

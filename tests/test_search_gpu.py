@@ -206,6 +206,71 @@ def test_small_entry_degree_fills_with_none():
     assert (ids[:, 2:] == NONE).all() and np.isinf(d[:, 2:]).all()
 
 
+def _components_graph(rng, sizes, r_pad):
+    """Disconnected graph: nodes split into components of the given sizes,
+    each a random graph over its own nodes with degrees from 0 up, some rows
+    holding their own id (a self-loop).  Returns (adj, list of node arrays)."""
+    num = int(sum(sizes))
+    perm = rng.permutation(num).astype(np.uint32)
+    adj = np.full((num, r_pad), NONE, dtype=np.uint32)
+    comps, at = [], 0
+    for size in sizes:
+        nodes = perm[at:at + size]
+        at += size
+        comps.append(nodes)
+        for n in nodes:
+            d = int(rng.integers(0, min(r_pad, size) + 1))
+            row = rng.choice(nodes, size=d, replace=False)
+            if d and rng.random() < 0.2 and n not in row:
+                row[0] = n
+            adj[n, :d] = row
+    return adj, comps
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_disconnected_components_equal_oracle(seed):
+    """Entry points in components of 1 to 700 nodes, rows of degree 0 and
+    self-loops: a search ends once the reachable nodes are all expanded,
+    with fewer than K (or L) of them in the pool; both pool kernels, teams of
+    4 and 8, every pool layout and visited-table shape give the oracle's
+    ids, distance bits, distance and hop counts, SS_NONE past the filled
+    pool (src/annsearch.cpp:182-259 ends the same way)."""
+    import oracle
+    from scattersearch_amd._lib import (SS_POOL_LDS, SS_POOL_REGISTERS, SS_POOL_SHIFT,
+                                        SS_VISITED_TWO_CHOICE)
+    rng = np.random.default_rng(100 + seed)
+    r_pad = int(rng.choice([8, 16, 32]))
+    sizes = [1, 1, 3, 9, 40, 300, 700]
+    adj, comps = _components_graph(rng, sizes, r_pad)
+    num = adj.shape[0]
+    dim = int(rng.choice([16, 100]))
+    base = rng.standard_normal((num, dim)).astype(np.float32)
+    q = rng.standard_normal((24, dim)).astype(np.float32)
+    if seed == 0:  # one isolated node without its self-loop: an empty first pool
+        adj[comps[0][0]] = NONE
+    shapes = [dict(), dict(team_size=4), dict(team_size=8),
+              dict(pool_layout=SS_POOL_REGISTERS), dict(pool_layout=SS_POOL_LDS),
+              dict(pool_layout=SS_POOL_SHIFT, team_size=4, waves_per_block=1),
+              dict(visited_slots=2048, visited_bucket=16 | SS_VISITED_TWO_CHOICE)]
+    for c, nodes in enumerate(comps):
+        ep = int(nodes[rng.integers(0, nodes.size)])
+        ix = _index(base, adj, ep)
+        for L in (4, 16, 70, 600):
+            K = min(L, 10)
+            ora = oracle.search_batch(base, adj, ep, q, K, L)
+            for shape in shapes:
+                if shape.get("pool_layout") == SS_POOL_REGISTERS and L > 64:
+                    continue
+                what = f"seed={seed} component of {nodes.size} L={L} {shape}"
+                print(what, flush=True)
+                gpu = ix.search(q, K, L, **shape)
+                exact = (gpu[2]["lossy"] == 0).all()
+                _assert_same(gpu, ora, check_ndc=exact, what=what)
+                filled = (ora[0] != NONE).sum(1)
+                assert (filled <= nodes.size).all()
+                assert np.isinf(gpu[1][ora[0] == NONE]).all(), what
+
+
 def test_hipgraph_replay_matches_eager(small):
     ix = _index(small["base"], small["adj"], small["ep"])
     eager = ix.search(small["queries"], 10, 48)
