@@ -253,10 +253,8 @@ __global__ __launch_bounds__(64) void search_large_kernel(SearchArgs a)
     while (true) {
         uint32_t q;
         if (a.queue) {
-            uint32_t pos = 0;
-            if (lane_id() == 0)
-                pos = atomicAdd(a.queue, 1u);
-            pos = uniform(pos);
+            const uint32_t pos = next_position(a.queue, first, 0u, 1u, false);  // one wavefront per block
+            first = false;
             if (pos >= rcnt)
                 break;
             q = a.order ? uniform(a.order[rbeg + pos]) : rbeg + pos;
