@@ -307,7 +307,9 @@ int ss_nn_descent(ss_index *index, uint32_t k, uint32_t iters, uint32_t seed, ui
  * max_recall_loss
  * = 0 keeps exact results.  The choice is stored in the index for L's
  * power-of-two bucket (dimension and degree are the index's) and used by
- * later searches that leave those fields 0. */
+ * later searches that leave those fields 0 and either leave expand_width and
+ * waves_per_query 0 or ask for the tuned ones (a search of another form
+ * takes the defaults). */
 int ss_autotune(ss_index *index, const float *queries, uint32_t nq, uint32_t K, uint32_t L,
                 float max_recall_loss, ss_search_params *best, float *best_ms,
                 float *best_recall);

@@ -536,6 +536,11 @@ static ss_search_params resolve_params(const ss_index *ix, uint32_t l_max, const
     for (const auto &t : ix->tuned) {
         if (t.l_bucket != bucket)
             continue;
+        // A tuned shape serves launches that leave the search form to it or
+        // ask for the same one (its team size and layouts belong to it).
+        if ((r.expand_width && r.expand_width != t.expand_width) ||
+            (r.waves_per_query && r.waves_per_query != t.waves_per_query))
+            continue;
         if (!r.expand_width)
             r.expand_width = t.expand_width;
         if (!r.waves_per_block)

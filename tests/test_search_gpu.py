@@ -366,6 +366,11 @@ def test_autotune_exact_and_relaxed(small):
     exact = ix.search(small["queries"], 10, 48, expand_width=1)
     best = ix.autotune(small["queries"], 10, 48, max_recall_loss=0.0)
     assert best["expand_width"] == 1 and best["ms"] > 0
+    # a search of another form does not take the tuned shape's team size
+    # (teams of 8 only run one node per step on one wavefront)
+    for kw in (dict(expand_width=2), dict(waves_per_query=2)):
+        ids = ix.search(small["queries"], 10, 48, **kw)[0]
+        assert (ids[:, 0] != NONE).all(), kw
     tuned = ix.search(small["queries"], 10, 48)  # parameters left at 0 take the tuned ones
     assert np.array_equal(tuned[0], exact[0])
     gt, _ = oracle.bruteforce(small["base"], small["queries"], 10)
