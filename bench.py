@@ -434,7 +434,9 @@ def main():
         "l_sweep": sweep,
         "graph": {"mean_degree": ginfo["passes"][-1]["mean_degree"], "knn_s": round(ginfo["knn_s"], 1),
                   "passes": len(ginfo["passes"])},
-        "roofline": roofline_record(w.name, nq, K, L, order_used(nq), achieved_gbs, launch_bytes, kern_ms),
+        "roofline": roofline_record(w.name, nq, K, L, order_used(nq), achieved_gbs, launch_bytes, kern_ms,
+                                    match={"visited_slots": args.visited_slots or tuned["visited_slots"],
+                                           "visited_bucket": args.visited_bucket or tuned["visited_bucket"]}),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -470,28 +472,31 @@ def exact_ndc(ids, st, launch_with_slots, d_ids, stats_host):
 def measured_traffic(workload, nq, K, L, order, match=None):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
     calibrated against a plain copy) for this workload and launch, from the
-    record kept in profiles/search_kernel_<workload>_counters.md; None when
-    there is no record for exactly this launch (workload, batch, K, L,
-    query order and the fields of `match`)."""
+    records kept in profiles/search_kernel_<workload>_counters.md (one JSON
+    block per launch shape measured): the record of exactly this launch
+    (workload, batch, K, L, query order and the fields of `match`), else one
+    of the same workload, batch, K, L and order measured at another shape
+    (returned with "shape_differs" set), else None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"search_kernel_{workload}_counters.md")
     try:
         text = open(path).read()
     except OSError:
         return None
-    block = text.split("```json", 1)
-    if len(block) < 2:
-        return None
-    try:
-        rec = json.loads(block[1].split("```", 1)[0])
-    except ValueError:
-        return None
-    if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L"), rec.get("query_order", "index")) != \
-            (workload, nq, K, L, order):
-        return None
-    if any(rec.get(k) != v for k, v in (match or {}).items()):
-        return None
-    return rec
+    near = None
+    for block in text.split("```json")[1:]:
+        try:
+            rec = json.loads(block.split("```", 1)[0])
+        except ValueError:
+            continue
+        if (rec.get("workload"), rec.get("nq"), rec.get("K"), rec.get("L"), rec.get("query_order", "index")) != \
+                (workload, nq, K, L, order):
+            continue
+        if any(rec.get(k) != v for k, v in (match or {}).items()):
+            near = near or dict(rec, shape_differs=True)
+            continue
+        return rec
+    return near
 
 
 def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_ms, match=None):
@@ -503,6 +508,9 @@ def roofline_record(workload, nq, K, L, order, achieved_gbs, launch_bytes, kern_
     if rec:
         out["traffic_note"] = (f"rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
                                f"profiles/search_kernel_{workload}_counters.md; L2 hit rate {rec['l2_hit_rate']}")
+        if rec.get("shape_differs"):
+            shape = {k: rec.get(k) for k in (match or {})}
+            out["traffic_note"] += f"; measured at {shape}, this launch {match}"
     return out
 
 
