@@ -119,6 +119,40 @@ def test_launch_shapes_same_results(small, wpb):
     assert np.array_equal(ref[2]["ndc"], out[2]["ndc"])
 
 
+def test_grid_sizes_take_every_query_once(small):
+    """Each wavefront takes its first query from its place in the grid (blocks
+    numbered XCD by XCD) and later ones from the launch's counter: grids of
+    1 to 1,000 blocks, not multiples of 8, larger and smaller than the batch,
+    in index and locality order, for both pool kernels and for pool-size
+    classes (each class a range of the order), all return the oracle's
+    result for every query."""
+    import oracle
+    from scattersearch_amd._lib import SS_ORDER_INDEX, SS_ORDER_LOCALITY
+    ix = _index(small["base"], small["adj"], small["ep"])
+    q = np.resize(small["queries"], (150, small["queries"].shape[1]))
+    rng = np.random.default_rng(21)
+    Ku = rng.choice([1, 10], size=q.shape[0]).astype(np.uint32)
+    Lu = np.maximum(Ku, rng.integers(8, 300, size=q.shape[0])).astype(np.uint32)
+    want = {L: oracle.search_batch(small["base"], small["adj"], small["ep"], q, 10, L) for L in (48, 600)}
+    want_u = oracle.search_batch(small["base"], small["adj"], small["ep"], q, Ku, Lu, k_stride=10)
+    for blocks in (1, 3, 7, 9, 13, 100, 1000):
+        for order in (SS_ORDER_INDEX, SS_ORDER_LOCALITY):
+            for wpb in (1, 4):
+                what = f"blocks={blocks} order={order} wpb={wpb}"
+                print(what, flush=True)
+                _assert_same(ix.search(q, 10, 48, blocks=blocks, query_order=order, waves_per_block=wpb),
+                             want[48], what=what + " L=48")
+            big = ix.search(q, 10, 600, blocks=blocks, query_order=order)
+            _assert_same(big, want[600], check_ndc=False, what=f"blocks={blocks} order={order} L=600")
+            exact = big[2]["lossy"] == 0  # the large-pool table may forget ids: only NDC grows
+            assert np.array_equal(big[2]["ndc"][exact], want[600][2][exact]), (blocks, order)
+            got = ix.search(q, Ku, Lu, k_stride=10, blocks=blocks, query_order=order)
+            for i in range(q.shape[0]):
+                k = int(Ku[i])
+                assert np.array_equal(got[0][i, :k], want_u[0][i, :k]), (blocks, order, i)
+            assert np.array_equal(got[2]["hops"], want_u[3]), (blocks, order)
+
+
 def test_uneven_k_and_l_in_one_launch(small):
     import oracle
     rng = np.random.default_rng(3)
