@@ -692,8 +692,10 @@ __global__ __launch_bounds__(512, scatter_min_waves<NBLK>()) void scatter_kernel
     bool first = true;
     while (true) {
         if (a.queue) {
+            // the first query of a block from its place in the grid, the
+            // later ones from the counter (as search_kernel takes them)
             if (threadIdx.x == 0)
-                s.ctl->q = atomicAdd(a.queue, 1u);
+                s.ctl->q = next_position(a.queue, first, 0u, 1u, true);
         } else if (threadIdx.x == 0) {
             s.ctl->q = first ? blockIdx.x : 0xFFFFFFFFu;
         }
