@@ -138,6 +138,43 @@ def test_locality_order_graph_replays(big):
         assert mid[0].shape == (50, 5)
 
 
+def test_orders_of_a_batch_sorted_in_several_passes(small):
+    """Batches above 16,384 queries are sorted by the order kernel in several
+    passes over the keys: 40,000 distinct queries in the locality order, per
+    query K and L split into pool-size classes, and one kernel longest pool
+    first all equal the index order, and a sample equals the oracle."""
+    import oracle
+    from scattersearch_amd import Index
+    from scattersearch_amd._lib import SS_CLASSES_OFF, SS_ORDER_INDEX, SS_ORDER_LOCALITY
+    rng = np.random.default_rng(40)
+    ix = Index(small["base"], 0)
+    ix.set_graph(small["adj"], small["ep"])
+    nq = 40_000
+    base_q = small["queries"]
+    q = (base_q[rng.integers(0, base_q.shape[0], size=nq)]
+         + rng.normal(0, 0.05, size=(nq, base_q.shape[1]))).astype(np.float32)
+    print(f"locality order, {nq} queries", flush=True)
+    a = ix.search(q, 10, 48, query_order=SS_ORDER_LOCALITY)
+    b = ix.search(q, 10, 48, query_order=SS_ORDER_INDEX)
+    _same(a, b, "locality vs index order, 40,000 queries")
+    sel = rng.choice(nq, 300, replace=False)
+    o_ids, o_d, o_ndc, o_hops = oracle.search_batch(small["base"], small["adj"], small["ep"], q[sel], 10, 48)
+    assert np.array_equal(a[0][sel], o_ids)
+    assert np.array_equal(a[1][sel].view(np.uint32), o_d.view(np.uint32))
+    assert np.array_equal(a[2]["hops"][sel], o_hops)
+    K = rng.choice([1, 10], size=nq).astype(np.uint32)
+    L = np.maximum(K, rng.integers(8, 300, size=nq)).astype(np.uint32)
+    print("pool-size classes, 40,000 queries", flush=True)
+    c = ix.search(q, K, L, k_stride=10)
+    d = ix.search(q, K, L, k_stride=10, query_order=SS_ORDER_INDEX)
+    _same(c, d, "classes vs index order")
+    L2 = np.maximum(K, rng.integers(65, 300, size=nq)).astype(np.uint32)
+    print("longest pool first, 40,000 queries", flush=True)
+    e = ix.search(q, K, L2, k_stride=10, pool_classes=SS_CLASSES_OFF)
+    f = ix.search(q, K, L2, k_stride=10, pool_classes=SS_CLASSES_OFF, query_order=SS_ORDER_INDEX)
+    _same(e, f, "longest first vs index order")
+
+
 @pytest.mark.parametrize("nq", [1, 3, 7, 8, 9, 64, 1000])
 def test_forced_locality_on_small_batches(small, nq):
     """Small and odd batch sizes, one query up to less than the default threshold."""
