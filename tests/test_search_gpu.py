@@ -225,6 +225,24 @@ def test_random_graphs_with_ties(seed):
         _assert_same(gpu, ora, what=f"seed={seed} L={L}")
 
 
+@pytest.mark.parametrize("K,L", [(1, 1), (1, 2), (2, 2), (64, 64), (65, 65), (512, 512), (512, 1024)])
+def test_extreme_k_and_l_equal_oracle(c0, K, L):
+    """The smallest pools, K = L at chunk boundaries, and K at its maximum
+    (SS_MAX_K = 512) for the register, LDS and large-pool kernels."""
+    import oracle
+    ix = _index(c0["base"], c0["adj"], c0["ep"])
+    q = c0["queries"][:24]
+    print(f"K={K} L={L}", flush=True)
+    ids, dists, st = ix.search(q, K, L, visited_slots=8192)
+    o_ids, o_d, o_ndc, o_hops = oracle.search_batch(c0["base"], c0["adj"], c0["ep"], q, K, L)
+    assert np.array_equal(ids, o_ids), f"K={K} L={L}: ids differ"
+    assert np.array_equal(dists.view(np.uint32), o_d.view(np.uint32)), f"K={K} L={L}: distances differ"
+    assert np.array_equal(st["hops"], o_hops)
+    exact = st["lossy"] == 0
+    assert np.array_equal(st["ndc"][exact], o_ndc[exact])
+    assert (st["status"] == 0).all()
+
+
 def test_small_entry_degree_fills_with_none():
     """deg(ep) < K: only improving candidates grow the pool, unfilled slots
     come back as (SS_NONE, +inf)."""
