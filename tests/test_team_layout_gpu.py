@@ -44,7 +44,8 @@ def test_teams_and_layouts_equal_oracle_c0(c0, team):
             _same(gpu, ora, f"team {team} layout {layout} L={L}")
 
 
-@pytest.mark.parametrize("dim,metric", [(200, 1), (40, 0), (96, 0)])
+@pytest.mark.parametrize("dim,metric", [(200, 1), (40, 0), (96, 0), (8, 0), (8, 1), (20, 1), (24, 0), (72, 1),
+                                        (136, 0), (520, 1), (1000, 0)])
 def test_team8_equals_oracle_other_dims(dim, metric):
     import oracle
     from scattersearch_amd import data
